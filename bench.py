@@ -1,0 +1,180 @@
+"""SIMPLEC outer iterations per second on one MI355X.
+
+Workload: the heated 4:1 duct, 1000 x 500 structured cells split into 1,000,000 triangles
+with interior nodes jittered by +-20 % of the spacing (seed 0), parabolic inlet u = 4 y (1-y),
+Re = 500, energy on (inlet T = 300, wall heat flux 5000, Pr = 0.7), FP64 SIMPLEC with
+SA-AMG-PCG (pressure tolerance 0.1) and Jacobi-BiCGSTAB. A step is one outer iteration; the
+AMG hierarchy is rebuilt every 10 iterations inside the timed region, as the solver does.
+
+Prints one JSON line (see ``--help``). ``roofline`` is measured live for the fused momentum
+assembly kernel (its launches bracketed by HIP events on the solver's stream), and
+``cpu_baseline`` times the CPU FP64 restatement of the same algorithm on a bounded sample
+of the same workload (rank 0, N = 1 only).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _workload(nx, ny):
+    from gputest_amd import case as C
+    from gputest_amd.mesh import duct_mesh
+
+    def parabolic(x):
+        import numpy as np
+
+        return np.stack([4.0 * x[:, 1] * (1.0 - x[:, 1]), np.zeros(x.shape[0])], axis=1)
+
+    mu = 1.0 / 500.0
+    B = C.BoundarySpec
+    case = C.UnstructuredCase(
+        boundaries={"inlet": B("inlet", velocity=[1.0, 0.0], value=300.0, profile=parabolic),
+                    "outlet": B("outlet", value=300.0),
+                    "bottom": B("wall", thermal="flux", value=5000.0),
+                    "top": B("wall", thermal="flux", value=5000.0)},
+        viscosity=mu, energy=True, conductivity=mu / 0.7, cp=1.0, init_T=300.0, pressure_tol=0.1,
+        max_iter=10 ** 9, tol=0.0)
+    return duct_mesh(nx, ny), case, parabolic
+
+
+def _assemble_bytes(s) -> int:
+    """Compulsory HBM bytes of one momentum-assembly launch (each array once)."""
+    fv, dim, nc, nf = s.fv, s.dim, s.nc, s.fv.n_if
+    cell_in = (nc + 1) * 4 * 2 + nc * 8 * (dim + dim * dim + dim + 1)  # cptr, bptr, U, gU, gp, vol
+    cell_out = nc * 8 * (3 + 2 * dim + 1)  # diag, diag_r, inv_r, b, x0, dP
+    half_edge = 2 * nf * (4 + 4 + 8)  # cface, cnbr in; off out
+    face_in = nf * 8 * (3 + 3 * dim)  # F, alpha, w; Tvec, dP_f, dN_f
+    nb = fv.n_bf
+    bnd = nb * (4 + 4 + 8 * (3 + dim))  # bface, kind, Fb, alpha_b, ub
+    return int(cell_in + cell_out + half_edge + face_in + bnd)
+
+
+def measure_roofline(s, reps=30):
+    """Average duration of the fused momentum-assembly kernel, timed with HIP events around
+    ``reps`` back-to-back launches on the solver's (current) stream."""
+    import torch
+
+    from gputest_amd import _native as nat
+
+    L = nat.lib()
+    fv, dim, c = s.fv, s.dim, s.case
+    a = nat.FtmiAssembleArgs()
+    a.F, a.Fb, a.gamma, a.energy, a.nfields = nat.ptr(s.F), nat.ptr(s.Fb), float(c.viscosity), 0, dim
+    for k in range(dim):
+        a.phi[k], a.grad[k], a.phib[k] = nat.ptr(s.U[k]), nat.ptr(s.gU[k]), nat.ptr(s.ub[k])
+        a.b[k], a.x0[k] = nat.ptr(s.bmom[k]), nat.ptr(s.Ustar[k])
+    a.gp, a.has_drive, a.relax, a.second_order, a.simplec, a.cp = nat.ptr(s.gp), 0, float(s.relax_u), 1, 1, 1.0
+    a.diag, a.diag_r, a.off, a.dP = nat.ptr(s.diag), nat.ptr(s.diag_r), nat.ptr(s.off), nat.ptr(s.dP)
+    a.resid, a.inv_r = nat.ptr(s.scal), nat.ptr(s.inv_r)
+    M, B, R = ctypes.byref(fv.c_mesh), ctypes.byref(s.c_bc), ctypes.byref(s.c_red)
+    st = nat.stream_ptr()
+    for _ in range(3):
+        nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, st), "assemble")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, st), "assemble")
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    nbytes = _assemble_bytes(s)
+    gbs = nbytes / t / 1e9
+    return {"kernel": "k_assemble<2,2> (fused momentum assembly)", "bound": "hbm", "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+            "bytes_per_launch": nbytes, "us_per_launch": round(t * 1e6, 2)}
+
+
+def cpu_baseline(nx, ny, iters):
+    """The oracle (CPU FP64 NumPy/SciPy restatement) on the same workload: set-up excluded,
+    one warm-up iteration (builds the first AMG hierarchy), then ``iters`` timed iterations."""
+    import numpy as np  # noqa: F401
+
+    import oracle.simplec_oracle as O
+
+    mesh, _, parabolic = _workload(nx, ny)
+    mu = 1.0 / 500.0
+    Bo = O.Boundary
+    case = O.Case(boundaries={"inlet": Bo("inlet", velocity=[1.0, 0.0], value=300.0, profile=parabolic),
+                              "outlet": Bo("outlet", value=300.0),
+                              "bottom": Bo("wall", thermal="flux", value=5000.0),
+                              "top": Bo("wall", thermal="flux", value=5000.0)},
+                  viscosity=mu, energy=True, conductivity=mu / 0.7, init_T=300.0, pressure_tol=0.1,
+                  max_iter=10 ** 9, tol=0.0)
+    s = O.Simplec(mesh, case)
+    s.step()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        s.step()
+    dt = time.perf_counter() - t0
+    return {"value": round(iters / dt, 4), "unit": "it/s", "cores": 1, "kind": "port",
+            "sample": f"{iters} outer iterations of the same {mesh.n_cells}-cell duct after 1 warm-up iteration "
+                      f"(AMG rebuilt every 10 iterations; NumPy/SciPy, single-threaded)"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nx", type=int, default=1000)
+    ap.add_argument("--ny", type=int, default=500)
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the SIMPLEC path on a single MI355X "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from gputest_amd.simple import SimpleSolver
+
+    mesh, case, _ = _workload(args.nx, args.ny)
+    t0 = time.perf_counter()
+    s = SimpleSolver(mesh, case, "cuda")
+    setup = time.perf_counter() - t0
+    print(f"[bench] set-up {setup:.1f} s: {s.nc} cells, {s.fv.n_if} interior faces", file=sys.stderr, flush=True)
+    for i in range(args.warmup):
+        s.step()
+    print(f"[bench] warm-up done: {s.last_linear}", file=sys.stderr, flush=True)
+    for k in s.timers:
+        s.timers[k] = 0.0
+    s.amg_setup_seconds = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        s.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"[bench] timed {args.steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
+          f"{s.amg_setup_seconds:.3f} s; last {s.last_linear}", file=sys.stderr, flush=True)
+    roof = measure_roofline(s)
+    out = {"metric": "SIMPLEC outer it/s at 1M cells (FP64)", "value": round(args.steps / dt, 3), "unit": "it/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded jittered-triangle mesh generated on the box)",
+           "config": {"workload": f"heated duct {args.nx}x{args.ny}x2 jittered triangles ({s.nc} cells), "
+                                  f"FP64 SIMPLEC + energy, SA-AMG-PCG tol 0.1",
+                      "cells": s.nc, "interior_faces": s.fv.n_if, "precision": "fp64",
+                      "stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
+                      "amg_setup_seconds": round(s.amg_setup_seconds, 3)},
+           "roofline": roof}
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_iters)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

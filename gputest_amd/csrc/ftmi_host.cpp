@@ -1,0 +1,74 @@
+// Host-side set-up helpers of libftmi (plain C++; called once per mesh or per AMG rebuild).
+#include <cstdint>
+#include <vector>
+
+#include "../../include/ftmi.h"
+
+extern "C" {
+
+// Per-row source lists for ftmi_assemble: row c receives (face << 1 | which, slot) pairs for
+// every face coefficient that lands in one of its slots, in face order (the order of the
+// reference's sequential loop, flowtrace/native/fvm_kernels.cpp:139-143).
+int ftmi_assemble_plan_host(int64_t nc, int64_t nf, const int32_t* indptr, const int32_t* slot_PN,
+                            const int32_t* slot_NP, int32_t* src_ptr, int32_t* src) {
+    // row of a slot: binary search in indptr
+    auto row_of = [&](int32_t slot) {
+        int64_t lo = 0, hi = nc;
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (indptr[mid] <= slot)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return lo;
+    };
+    std::vector<int32_t> cnt(nc + 1, 0);
+    for (int64_t f = 0; f < nf; ++f) {
+        cnt[row_of(slot_PN[f]) + 1] += 2;
+        cnt[row_of(slot_NP[f]) + 1] += 2;
+    }
+    for (int64_t c = 0; c < nc; ++c) cnt[c + 1] += cnt[c];
+    for (int64_t c = 0; c <= nc; ++c) src_ptr[c] = cnt[c];
+    std::vector<int32_t> pos(cnt.begin(), cnt.end() - 1);
+    for (int64_t f = 0; f < nf; ++f) {
+        const int64_t r1 = row_of(slot_PN[f]);
+        src[pos[r1]++] = (int32_t)(f << 1);
+        src[pos[r1]++] = slot_PN[f];
+        const int64_t r2 = row_of(slot_NP[f]);
+        src[pos[r2]++] = (int32_t)((f << 1) | 1);
+        src[pos[r2]++] = slot_NP[f];
+    }
+    return 0;
+}
+
+// Three-pass greedy aggregation (flowtrace/linalg/amg.py:26-57): seeds are nodes whose strong
+// neighbourhood is still free; leftovers join the first aggregated strong neighbour; what is
+// left becomes a singleton. Sequential by definition: the result depends on the node order.
+int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* idx, int64_t* agg) {
+    int64_t na = 0;
+    for (int64_t i = 0; i < n; ++i) agg[i] = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool all_free = true;
+        for (int64_t k = ptr[i]; k < ptr[i + 1] && all_free; ++k) all_free = agg[idx[k]] < 0;
+        if (!all_free) continue;
+        agg[i] = na;
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) agg[idx[k]] = na;
+        ++na;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+            if (agg[idx[k]] >= 0) {
+                agg[i] = agg[idx[k]];
+                break;
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (agg[i] < 0) agg[i] = na++;
+    return na;
+}
+
+}  // extern "C"

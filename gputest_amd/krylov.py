@@ -1,0 +1,95 @@
+"""Device-resident Krylov solvers (``flowtrace/linalg/krylov.py``) on HIP kernels.
+
+``bicgstab_jacobi`` is the reference's native entry (``flowtrace/native/__init__.py:220-237``)
+on a device CSR matrix; ``bicgstab_ldu`` and ``pcg_ldu`` run on the face (LDU) form the
+SIMPLEC stages assemble. All return ``(x, KrylovInfo)`` like the reference.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import torch
+
+from . import _native as nat
+
+
+@dataclass
+class KrylovInfo:
+    iterations: int
+    residual: float  # final relative residual
+    converged: bool
+
+
+class Workspace:
+    """Reusable device work buffers keyed by (kind, n)."""
+
+    def __init__(self, device):
+        self.device = torch.device(device)
+        self._bufs = {}
+
+    def get(self, kind: str, n: int) -> torch.Tensor:
+        key = (kind, n)
+        t = self._bufs.get(key)
+        if t is None:
+            L = nat.lib()
+            nbytes = L.ftmi_bicgstab_work_bytes(n) if kind == "bicgstab" else L.ftmi_pcg_work_bytes(n)
+            t = torch.zeros(int(nbytes), dtype=torch.uint8, device=self.device)
+            self._bufs[key] = t
+        return t
+
+
+_default_ws = {}
+
+
+def _ws(device) -> Workspace:
+    key = str(device)
+    if key not in _default_ws:
+        _default_ws[key] = Workspace(device)
+    return _default_ws[key]
+
+
+def _run(fn, *args):
+    it, res = ctypes.c_int64(0), ctypes.c_double(0.0)
+    st = fn(*args, ctypes.byref(it), ctypes.byref(res), nat.stream_ptr())
+    nat.check(st, "krylov")
+    return KrylovInfo(int(it.value), float(res.value), bool(st == 1))
+
+
+def bicgstab_jacobi(A, b, x0, inv_diag, tol: float, maxiter: int, relative_to_initial: bool = True, ws=None):
+    """Jacobi-right-preconditioned BiCGSTAB on a ``DeviceCSR`` (``krylov.py:68-121`` rules)."""
+    x = x0.clone().contiguous()
+    n = b.shape[0]
+    work = (ws or _ws(b.device)).get("bicgstab", n)
+    info = _run(nat.lib().ftmi_bicgstab_jacobi, n, nat.ptr(A.indptr), nat.ptr(A.indices), nat.ptr(A.data),
+                nat.ptr(b.contiguous()), nat.ptr(inv_diag.contiguous()), nat.ptr(x), float(tol), int(maxiter),
+                int(bool(relative_to_initial)), nat.ptr(work))
+    return x, info
+
+
+def bicgstab_ldu(fv, diag, off, b, inv_diag, x, tol: float, maxiter: int, relative_to_initial=True, ws=None):
+    """In place on ``x`` (x0 in, solution out); the operator is (diag, off) on ``fv``'s adjacency."""
+    n = b.shape[0]
+    work = (ws or _ws(b.device)).get("bicgstab", n)
+    return _run(nat.lib().ftmi_bicgstab_ldu, n, nat.ptr(diag), nat.ptr(fv.cptr), nat.ptr(fv.cnbr), nat.ptr(off),
+                nat.ptr(b), nat.ptr(inv_diag), nat.ptr(x), float(tol), int(maxiter), int(bool(relative_to_initial)),
+                nat.ptr(work))
+
+
+def pcg_ldu(fv, diag, off, b, x, tol: float, maxiter: int, amg=None, project=False, ws=None):
+    """PCG from zero into ``x``, preconditioned by one V-cycle of ``amg`` (``krylov.py:25-65``)."""
+    n = b.shape[0]
+    work = (ws or _ws(b.device)).get("pcg", n)
+    return _run(nat.lib().ftmi_pcg_ldu, n, nat.ptr(diag), nat.ptr(fv.cptr), nat.ptr(fv.cnbr), nat.ptr(off),
+                nat.ptr(b), nat.ptr(x), float(tol), int(maxiter), amg.handle if amg is not None else None,
+                int(bool(project)), nat.ptr(work))
+
+
+def pcg_csr(A, b, tol: float, maxiter: int, amg=None, project=False, ws=None):
+    n = b.shape[0]
+    x = torch.empty_like(b)
+    work = (ws or _ws(b.device)).get("pcg", n)
+    info = _run(nat.lib().ftmi_pcg_csr, n, nat.ptr(A.indptr), nat.ptr(A.indices), nat.ptr(A.data),
+                nat.ptr(b.contiguous()), nat.ptr(x), float(tol), int(maxiter),
+                amg.handle if amg is not None else None, int(bool(project)), nat.ptr(work))
+    return x, info

@@ -1,0 +1,522 @@
+"""Pressure-based SIMPLE/SIMPLEC outer iteration on one MI355X.
+
+``SimpleSolver`` keeps the reference's interface (``flowtrace/unstructured/simple.py:71-1072``:
+``step``, ``finished``, ``fields``, ``residuals``, ``state_dict``/``load_state``,
+``wall_forces``, ``heat_flow``, ``mass_balance``, ``describe``, ``timers``) and the same
+arithmetic, but one outer iteration is a short sequence of fused HIP kernels on HBM-resident
+fields:
+
+1. boundary values (``ftmi_simplec_bvals``);
+2. least-squares gradients of p and every velocity component in one pass, then the
+   Green-Gauss pressure force (``ftmi_ls_gradients``, ``ftmi_gg_gradient_fused``);
+3. momentum coefficients, right-hand sides, Fluent residual sums, relaxed system and the
+   SIMPLEC d-coefficient in one pass (``ftmi_transport_assemble``);
+4. one device-resident Jacobi-BiCGSTAB per component (``ftmi_bicgstab_ldu``);
+5. Rhie-Chow fluxes, pressure-correction matrix and mass imbalance (``ftmi_rhie_chow``);
+6. AMG-preconditioned PCG for p' (``ftmi_pcg_ldu``; hierarchy rebuilt every 10 iterations);
+7. flux, pressure and velocity corrections with the p' gradient (``ftmi_correct``);
+8. energy, when on (the same kernels for one scalar).
+
+The host reads one block of reduced scalars per iteration (residual sums, continuity, flux
+scale) and runs the reference's residual normalisation, convergence test and stability guard.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from collections import defaultdict
+from typing import Dict
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .amg import SmoothedAggregationAMG
+from .case import FLUENT_TO_KIND, BoundarySpec, UnstructuredCase
+from .fvmesh import FVMesh
+from .krylov import KrylovInfo, Workspace, bicgstab_ldu, pcg_ldu
+from .mesh import UnstructuredMesh
+
+WALL, INLET, OUTLET, PRESSURE, SYMMETRY = range(5)
+T_ADIABATIC, T_FIXED, T_FLUX = range(3)
+_KIND_CODE = {"wall": WALL, "inlet": INLET, "outlet": OUTLET, "pressure": PRESSURE, "symmetry": SYMMETRY}
+
+# slots of the per-iteration scalar block (device doubles)
+_S_MOM = 0  # 2 per component: sum |b - A u|, sum |diag u|
+_S_CONT = 6
+_S_SUMS = 7  # sum p_new, sum |F|, sum |Fb|
+_S_EN = 10  # energy: sum |b - A T|, sum |diag T|
+_NSCAL = 16
+
+
+def auto_relax_p(non_orthogonality_deg: float) -> float:
+    """SIMPLEC pressure relaxation when none is given (``simple.py:53-64``)."""
+    return 1.0 if non_orthogonality_deg <= 5.0 else 0.8
+
+
+class SimpleSolver:
+    GUARD_GROWTH = 20.0
+    GUARD_ROUNDOFF = 1e-6
+    GUARD_EVERY = 20
+    GUARD_FACTOR = 0.6
+    GUARD_MIN_RELAX = 0.1
+    GUARD_FALLBACK = True
+    GUARD_MIN_RELAX_U = 0.2
+
+    def __init__(self, mesh: UnstructuredMesh, case: UnstructuredCase, device="cuda"):
+        t0 = time.perf_counter()
+        self.case = case
+        if case.periodic and not mesh.periodic:
+            from .mesh import make_periodic
+
+            for za, zb in case.periodic:
+                mesh = make_periodic(mesh, za, zb)
+        self.mesh = mesh
+        self.device = torch.device(device)
+        nat.lib()  # fail now, not mid-iteration, when the HIP library is missing
+        fv = self.fv = FVMesh(mesh, device)
+        self.dim, self.nc = mesh.dim, mesh.n_cells
+        self.energy = case.energy
+        self.timers: Dict[str, float] = defaultdict(float)
+        self.iteration = 0
+        self.step_count = 0
+        self.converged = False
+        self._res_max5: Dict[str, float] = {}
+        self._amg = None
+        self._amg_age = 10 ** 9
+        self.amg_rebuild = 10
+        self.amg_setup_seconds = 0.0
+        self.last_linear = {}
+        self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
+        self._build_boundaries()
+        self._alloc()
+        U0 = np.zeros((self.nc, self.dim))
+        U0[:] = (list(case.init_velocity) + [0.0] * 3)[: self.dim]
+        self.U.copy_(torch.as_tensor(U0.T.copy(), device=self.device))
+        self.p.zero_()
+        if self.energy:
+            self.T.fill_(case.init_T)
+        self._has_lag = False
+        self._has_lagT = False
+        self._init_flux()
+        self.guard_enabled = case.relax_p is None
+        self._capture = None
+        self._reset_guard()
+        self.setup_seconds = time.perf_counter() - t0
+
+    # ------------------------------------------------------------------ set-up
+    def _build_boundaries(self):
+        fv, case, dim = self.fv, self.case, self.dim
+        n = fv.n_bf
+        kind = np.zeros(n, np.int32)
+        ub = np.zeros((n, dim))
+        pb = np.zeros(n)
+        tk = np.zeros(n, np.int32)
+        tv = np.zeros(n)
+        self.zone_specs = {}
+        for z in self.mesh.boundary_zones():
+            spec = case.boundaries.get(z.name)
+            if spec is None:
+                spec = BoundarySpec(kind=FLUENT_TO_KIND.get(z.type, "wall"))
+                if spec.kind == "inlet":
+                    spec.normal_speed = 1.0
+            self.zone_specs[z.name] = spec
+            m = fv.bzone == z.id
+            if spec.kind == "periodic" or z.type in ("periodic", "periodic-shadow"):
+                if m.any():
+                    raise ValueError(f"zone {z.name!r} is periodic but not paired")
+                continue
+            kind[m] = _KIND_CODE[spec.kind]
+            if spec.kind in ("wall", "inlet"):
+                if spec.profile is not None:
+                    ub[m] = np.asarray(spec.profile(fv.xfb_h[m]), float).reshape(-1, dim)
+                elif spec.velocity is not None:
+                    ub[m] = (list(spec.velocity) + [0.0] * 3)[:dim]
+                elif spec.normal_speed is not None:
+                    ub[m] = -spec.normal_speed * fv.nb_h[m]
+            pb[m] = spec.pressure
+            if spec.kind == "wall":
+                tk[m] = {"adiabatic": T_ADIABATIC, "temperature": T_FIXED, "flux": T_FLUX}[spec.thermal]
+            elif spec.kind in ("inlet", "pressure"):
+                tk[m] = T_FIXED
+            tv[m] = spec.value
+        self.bkind_h, self.ub_fixed_h, self.pb_fixed_h, self.tkind_h, self.tval_h = kind, ub, pb, tk, tv
+        dev = self.device
+        self.bkind = torch.as_tensor(kind, device=dev)
+        self.tkind = torch.as_tensor(tk, device=dev)
+        self.ub_fixed = torch.as_tensor(ub, device=dev)
+        self.pb_fixed = torch.as_tensor(pb, device=dev)
+        self.tval = torch.as_tensor(tv, device=dev)
+        self.singular_p = not bool(((kind == OUTLET) | (kind == PRESSURE)).any())
+        b = nat.FtmiBc()
+        b.kind, b.tkind = nat.ptr(self.bkind), nat.ptr(self.tkind)
+        b.ub_fixed, b.pb_fixed, b.tval = nat.ptr(self.ub_fixed), nat.ptr(self.pb_fixed), nat.ptr(self.tval)
+        self.c_bc = b
+
+    def _alloc(self):
+        nc, nf, nb, dim, dev = self.nc, self.fv.n_if, self.fv.n_bf, self.dim, self.device
+        z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)  # noqa: E731
+        self.U, self.p = z(dim, nc), z(nc)
+        self.F, self.Fb = z(nf), z(nb)
+        self.T = z(nc) if self.energy else None
+        self.ub, self.pb = z(dim, nb), z(nb)
+        self.gp_ls, self.gp, self.gU = z(nc, dim), z(nc, dim), z(dim, nc, dim)
+        self.diag, self.diag_r, self.inv_r, self.off = z(nc), z(nc), z(nc), z(2 * nf)
+        self.bmom, self.Ustar, self.dP = z(dim, nc), z(dim, nc), z(nc)
+        self.Fst, self.cf, self.Fbst, self.cb = z(nf), z(nf), z(nb), z(nb)
+        self.diag_p, self.offp, self.mass, self.rhs_p, self.pp = z(nc), z(2 * nf), z(nc), z(nc), z(nc)
+        self.gpp = z(nc, dim)
+        if self.energy:
+            self.Tb, self.gT, self.Tnew, self.bT = z(nb), z(nc, dim), z(nc), z(nc)
+            self.diagT, self.diagT_r = z(nc), z(nc)
+        self.scal = z(_NSCAL)
+        self.red_partials = z(2048 * 8)
+        self.red_counter = torch.zeros(64, dtype=torch.int32, device=dev)
+        r = nat.FtmiRed()
+        r.partials, r.counter = nat.ptr(self.red_partials), nat.ptr(self.red_counter)
+        self.c_red = r
+        self.ws = Workspace(dev)
+
+    def _init_flux(self):
+        L, s = nat.lib(), nat.stream_ptr()
+        fv = self.fv
+        nat.check(L.ftmi_simplec_bvals(ctypes.byref(fv.c_mesh), ctypes.byref(self.c_bc), nat.ptr(self.U),
+                                       nat.ptr(self.p), None, None, nat.ptr(self.ub), nat.ptr(self.pb), s), "bvals")
+        nat.check(L.ftmi_init_flux(ctypes.byref(fv.c_mesh), ctypes.byref(self.c_bc), float(self.case.density),
+                                   nat.ptr(self.U), nat.ptr(self.ub), nat.ptr(self.F), nat.ptr(self.Fb), s), "init_flux")
+
+    # ------------------------------------------------------------------ the outer iteration
+    @property
+    def finished(self) -> bool:
+        return self.converged or self.iteration >= self.case.max_iter
+
+    def _mark(self, name):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        self._events.append((name, ev))
+
+    def step(self) -> float:
+        if self.finished:
+            return 0.0
+        c, fv, dim = self.case, self.fv, self.dim
+        L, s = nat.lib(), nat.stream_ptr()
+        M, B, R = ctypes.byref(fv.c_mesh), ctypes.byref(self.c_bc), ctypes.byref(self.c_red)
+        rho, mu = c.density, c.viscosity
+        self._events = []
+        self._mark("start")
+        # 1-2. boundary values with the lagged gradients, then the new gradients
+        lag = self._has_lag
+        nat.check(L.ftmi_simplec_bvals(M, B, nat.ptr(self.U), nat.ptr(self.p), nat.ptr(self.gU) if lag else None,
+                                       nat.ptr(self.gp) if lag else None, nat.ptr(self.ub), nat.ptr(self.pb), s),
+                  "bvals")
+        phis = [self.p] + [self.U[k] for k in range(dim)]
+        phibs = [self.pb] + [self.ub[k] for k in range(dim)]
+        outs = [self.gp_ls] + [self.gU[k] for k in range(dim)]
+        arr = ctypes.c_void_p * len(phis)
+        nat.check(L.ftmi_ls_gradients(M, len(phis), arr(*[t.data_ptr() for t in phis]),
+                                      arr(*[t.data_ptr() for t in phibs]), arr(*[t.data_ptr() for t in outs]),
+                                      None, s), "ls_gradients")
+        if c.pressure_gradient == "green-gauss":
+            nat.check(L.ftmi_gg_gradient_fused(M, nat.ptr(self.p), nat.ptr(self.pb), nat.ptr(self.gp_ls),
+                                               nat.ptr(self.gp), s), "gg_gradient")
+        else:
+            self.gp.copy_(self.gp_ls)
+        self._has_lag = True
+        self._mark("gradients")
+        # 3-4. momentum
+        au = self.relax_u
+        drive = np.asarray(list(c.body_force)[:dim], float) - np.asarray(
+            (list(c.periodic_pressure_gradient) + [0.0] * 3)[:dim], float)
+        a = nat.FtmiAssembleArgs()
+        a.F, a.Fb, a.gamma, a.energy, a.nfields = nat.ptr(self.F), nat.ptr(self.Fb), float(mu), 0, dim
+        for k in range(dim):
+            a.phi[k], a.grad[k], a.phib[k] = nat.ptr(self.U[k]), nat.ptr(self.gU[k]), nat.ptr(self.ub[k])
+            a.b[k], a.x0[k] = nat.ptr(self.bmom[k]), nat.ptr(self.Ustar[k])
+            a.drive[k] = float(drive[k])
+        a.gp, a.has_drive, a.relax = nat.ptr(self.gp), int(bool(np.any(drive))), float(au)
+        a.second_order = int(c.scheme == "second-order")
+        a.simplec = 1 if self.algo == "SIMPLEC" else 0
+        a.cp = float(c.cp)
+        a.diag, a.diag_r, a.off, a.dP = nat.ptr(self.diag), nat.ptr(self.diag_r), nat.ptr(self.off), nat.ptr(self.dP)
+        a.resid, a.inv_r = nat.ptr(self.scal[_S_MOM:]), nat.ptr(self.inv_r)
+        nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "momentum assembly")
+        infos = []
+        for k in range(dim):
+            infos.append(bicgstab_ldu(fv, self.diag_r, self.off, self.bmom[k], self.inv_r, self.Ustar[k], 0.1, 30,
+                                      True, ws=self.ws))
+        self._mark("momentum")
+        # 5. Rhie-Chow fluxes and the pressure-correction system
+        nat.check(L.ftmi_rhie_chow(M, B, float(rho), nat.ptr(self.Ustar), nat.ptr(self.p), nat.ptr(self.gp),
+                                   nat.ptr(self.dP), nat.ptr(self.Fst), nat.ptr(self.cf), nat.ptr(self.Fbst),
+                                   nat.ptr(self.cb), nat.ptr(self.diag_p), nat.ptr(self.offp), nat.ptr(self.mass),
+                                   nat.ptr(self.rhs_p), nat.ptr(self.scal[_S_CONT:]), R, s), "rhie_chow")
+        self._mark("assembly")
+        # 6. pressure correction
+        if self._amg is None or self._amg_age >= self.amg_rebuild:
+            t0 = time.perf_counter()
+            Ah = fv.ldu_to_csr_host(self.diag_p.cpu().numpy(), self.offp.cpu().numpy())
+            self._amg = SmoothedAggregationAMG(Ah, self.device, singular=self.singular_p)
+            self._amg_age = 0
+            self.amg_setup_seconds += time.perf_counter() - t0
+        self._amg_age += 1
+        pinfo = pcg_ldu(fv, self.diag_p, self.offp, self.rhs_p, self.pp, c.pressure_tol, 200, self._amg,
+                        self.singular_p, ws=self.ws)
+        self._mark("pressure")
+        # 7. corrections
+        nat.check(L.ftmi_correct(M, B, nat.ptr(self.pp), nat.ptr(self.Ustar), nat.ptr(self.dP), nat.ptr(self.Fst),
+                                 nat.ptr(self.cf), nat.ptr(self.Fbst), nat.ptr(self.cb), float(self.relax_p),
+                                 int(self.singular_p), nat.ptr(self.U), nat.ptr(self.p), nat.ptr(self.F),
+                                 nat.ptr(self.Fb), nat.ptr(self.gpp), nat.ptr(self.scal[_S_SUMS:]), R, s), "correct")
+        # 8. energy
+        tinfo = None
+        if self.energy:
+            tinfo = self._energy_step(L, s, M, B, R)
+        self._mark("correct+energy")
+        sc = self.scal.cpu().numpy()  # the one host read of the iteration
+        self._accumulate_timers()
+        res = {}
+        num = [sc[_S_MOM + 2 * k] for k in range(dim)]
+        den = [sc[_S_MOM + 2 * k + 1] for k in range(dim)]
+        floor = 1e-3 * max(max(den), 1e-300)
+        for k, name in enumerate(("x-velocity", "y-velocity", "z-velocity")[:dim]):
+            res[name] = num[k] / max(den[k], floor)
+        res["continuity"] = float(sc[_S_CONT])
+        if self.energy:
+            r, sd = float(sc[_S_EN]), float(sc[_S_EN + 1])
+            res["energy"] = r / sd if sd > 0 else r
+        self.iteration += 1
+        self.step_count = self.iteration
+        self.last_linear = {"momentum_iters": [i.iterations for i in infos], "pressure_iters": pinfo.iterations,
+                            "pressure_residual": pinfo.residual}
+        if tinfo is not None:
+            self.last_linear["energy_iters"] = tinfo.iterations
+        self._capture = {"mom_info": infos, "p_info": pinfo, "energy_info": tinfo}
+        prev = getattr(self, "_last_res", {}), getattr(self, "_last_raw", {})
+        self._last_raw = dict(res)
+        if self.iteration <= 5:
+            self._res_max5["continuity"] = max(self._res_max5.get("continuity", 0.0), res["continuity"], 1e-300)
+        flux_scale = float(sc[_S_SUMS + 1]) + float(sc[_S_SUMS + 2])
+        res["continuity"] /= max(self._res_max5["continuity"], 1e-3 * flux_scale)
+        self._last_res = res
+        rolled_back = self._stability_guard(self._last_raw["continuity"], flux_scale)
+        if rolled_back:
+            self._last_res, self._last_raw = prev
+            return 1.0
+        if self.iteration > 5 and max(res.values()) < c.tol:
+            self.converged = True
+        return 1.0
+
+    def _energy_step(self, L, s, M, B, R):
+        c = self.case
+        nat.check(L.ftmi_energy_bvals(M, B, nat.ptr(self.T), nat.ptr(self.gT) if self._has_lagT else None,
+                                      nat.ptr(self.Fb), nat.ptr(self.Tb), s), "energy bvals")
+        arr = ctypes.c_void_p * 1
+        nat.check(L.ftmi_ls_gradients(M, 1, arr(self.T.data_ptr()), arr(self.Tb.data_ptr()),
+                                      arr(self.gT.data_ptr()), None, s), "T gradient")
+        self._has_lagT = True
+        a = nat.FtmiAssembleArgs()
+        a.F, a.Fb, a.gamma, a.energy, a.nfields = nat.ptr(self.F), nat.ptr(self.Fb), float(c.conductivity / c.cp), 1, 1
+        a.phi[0], a.grad[0], a.phib[0] = nat.ptr(self.T), nat.ptr(self.gT), nat.ptr(self.Tb)
+        a.b[0], a.x0[0] = nat.ptr(self.bT), nat.ptr(self.Tnew)
+        a.relax, a.second_order, a.simplec, a.cp = float(c.relax_T), int(c.scheme == "second-order"), -1, float(c.cp)
+        a.diag, a.diag_r, a.off = nat.ptr(self.diagT), nat.ptr(self.diagT_r), nat.ptr(self.off)
+        a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(self.inv_r)
+        nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
+        info = bicgstab_ldu(self.fv, self.diagT_r, self.off, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
+                            ws=self.ws)
+        self.T, self.Tnew = self.Tnew, self.T
+        return info
+
+    def _accumulate_timers(self):
+        evs = self._events
+        for (name, e1), (_, e0) in zip(evs[1:], evs[:-1]):
+            self.timers[name] += e0.elapsed_time(e1) * 1e-3
+
+    # ------------------------------------------------------------------ stability guard (simple.py:705-776)
+    def _reset_guard(self):
+        self.guard_events = []
+        self.algo = self.case.algorithm
+        self.relax_u = self.case.relax_u
+        self.guard_exhausted_at = None
+        self._guard_min = float("inf")
+        self._guard_snap = (self._device_snapshot(), float("inf")) if self.guard_enabled else None
+
+    def _device_snapshot(self):
+        st = {"U": self.U.clone(), "p": self.p.clone(), "F": self.F.clone(), "Fb": self.Fb.clone(),
+              "iteration": self.iteration, "res_continuity_norm": self._res_max5.get("continuity", 0.0)}
+        if self.energy:
+            st["T"] = self.T.clone()
+        return st
+
+    def _restore_snapshot(self, st):
+        self.U.copy_(st["U"])
+        self.p.copy_(st["p"])
+        self.F.copy_(st["F"])
+        self.Fb.copy_(st["Fb"])
+        if self.energy and "T" in st:
+            self.T.copy_(st["T"])
+        self.iteration = int(st["iteration"])
+        self._has_lag = self._has_lagT = False
+        if float(st.get("res_continuity_norm", 0.0)) > 0:
+            self._res_max5["continuity"] = float(st["res_continuity_norm"])
+
+    def _stability_guard(self, raw_continuity: float, flux_scale: float) -> bool:
+        if not self.guard_enabled:
+            return False
+        r = raw_continuity
+        ref = max(self._guard_min, self.GUARD_ROUNDOFF * flux_scale)
+        bad = not math.isfinite(r) or (self.iteration > 10 and r > self.GUARD_GROWTH * ref)
+        if bad and self._guard_snap is not None and self.relax_p * self.GUARD_FACTOR >= self.GUARD_MIN_RELAX:
+            snap, snap_r = self._guard_snap
+            at = self.iteration
+            self._restore_snapshot(snap)
+            self.iteration = self.step_count = at
+            self.relax_p *= self.GUARD_FACTOR
+            self._amg_age = 10 ** 9
+            self._guard_min = snap_r
+            self.guard_events.append({"detected_at": at, "restored_to": int(snap["iteration"]),
+                                      "relax_p": round(self.relax_p, 4), "continuity": r})
+            return True
+        stage2 = self.GUARD_FALLBACK and (self.algo == "SIMPLEC" or self.relax_u * 0.7 >= self.GUARD_MIN_RELAX_U - 1e-12)
+        if bad and self._guard_snap is not None and stage2:
+            snap, snap_r = self._guard_snap
+            at = self.iteration
+            self._restore_snapshot(snap)
+            self.iteration = self.step_count = at
+            if self.algo == "SIMPLEC":
+                self.algo, self.relax_u, self.relax_p = "SIMPLE", min(self.relax_u, 0.3), 0.3
+            else:
+                self.relax_u *= 0.7
+            self._amg_age = 10 ** 9
+            self._guard_min = snap_r
+            self.guard_events.append({"detected_at": at, "restored_to": int(snap["iteration"]), "algorithm": self.algo,
+                                      "relax_p": round(self.relax_p, 4), "relax_u": round(self.relax_u, 4),
+                                      "continuity": r})
+            return True
+        if bad and self.guard_exhausted_at is None:
+            self.guard_exhausted_at = self.iteration
+        if math.isfinite(r) and self.iteration > 5:
+            self._guard_min = min(self._guard_min, r)
+            if self.iteration % self.GUARD_EVERY == 0 and r <= 2.0 * self._guard_min:
+                self._guard_snap = (self._device_snapshot(), r)
+        return False
+
+    # ------------------------------------------------------------------ state, fields, reports
+    def residuals(self) -> dict:
+        r = dict(getattr(self, "_last_res", {}))
+        raw = dict(getattr(self, "_last_raw", r))
+        return {"raw": raw, "scaled": r, "normalized": r}
+
+    def fields(self, names=None) -> Dict[str, np.ndarray]:
+        U = self.U.cpu().numpy().T
+        out = {"x-velocity": U[:, 0].copy(), "y-velocity": U[:, 1].copy()}
+        if self.dim == 3:
+            out["z-velocity"] = U[:, 2].copy()
+        out["velocity-magnitude"] = np.linalg.norm(U, axis=1)
+        out["pressure"] = self.p.cpu().numpy()
+        if self.energy:
+            out["temperature"] = self.T.cpu().numpy()
+        if names:
+            out = {k: v for k, v in out.items() if k in names}
+        return out
+
+    def cell_centres(self) -> np.ndarray:
+        return self.fv.xc_h
+
+    def state_dict(self) -> dict:
+        st = {"U": self.U.cpu().numpy().T.copy(), "p": self.p.cpu().numpy(), "F": self.F.cpu().numpy(),
+              "Fb": self.Fb.cpu().numpy(), "iteration": self.iteration, "time": 0.0, "time_step": 0,
+              "res_continuity_norm": self._res_max5.get("continuity", 0.0)}
+        if self.energy:
+            st["T"] = self.T.cpu().numpy()
+        return st
+
+    def load_state(self, st: dict):
+        dev = self.device
+        self.U.copy_(torch.as_tensor(np.asarray(st["U"], float).T.copy(), device=dev))
+        self.p.copy_(torch.as_tensor(np.asarray(st["p"], float), device=dev))
+        self.F.copy_(torch.as_tensor(np.asarray(st["F"], float), device=dev))
+        self.Fb.copy_(torch.as_tensor(np.asarray(st["Fb"], float), device=dev))
+        if self.energy and "T" in st:
+            self.T.copy_(torch.as_tensor(np.asarray(st["T"], float), device=dev))
+        self.iteration = self.step_count = int(st.get("iteration", 0))
+        self._has_lag = self._has_lagT = False
+        self._capture = None
+        if float(st.get("res_continuity_norm", 0.0)) > 0:
+            self._res_max5["continuity"] = float(st["res_continuity_norm"])
+
+    def set_lagged_gradients(self, gp, gU, gT=None):
+        """Gradients of the previous iteration, used to reconstruct boundary values."""
+        dev = self.device
+        if gp is None:
+            self._has_lag = False
+        else:
+            self.gp.copy_(torch.as_tensor(np.asarray(gp, float), device=dev))
+            self.gU.copy_(torch.as_tensor(np.stack([np.asarray(g, float) for g in gU]), device=dev))
+            self._has_lag = True
+        if self.energy:
+            if gT is None:
+                self._has_lagT = False
+            else:
+                self.gT.copy_(torch.as_tensor(np.asarray(gT, float), device=dev))
+                self._has_lagT = True
+
+    def _pressure_bvals_h(self):
+        fv = self.fv
+        p = self.p.cpu().numpy()
+        pP = p[fv.Pb_h]
+        if self._has_lag:
+            pP = pP + np.sum(self.gp.cpu().numpy()[fv.Pb_h] * fv.db.cpu().numpy(), axis=1)
+        m_pres = ((self.bkind_h == OUTLET) | (self.bkind_h == PRESSURE)).astype(float)
+        return m_pres * self.pb_fixed_h + (1.0 - m_pres) * pP
+
+    def wall_forces(self) -> dict:
+        """Pressure + viscous force on every wall zone (``simple.py:970-1001``)."""
+        fv = self.fv
+        pb = self._pressure_bvals_h()
+        U = self.U.cpu().numpy().T
+        du = U[fv.Pb_h] - self.ub_fixed_h
+        du_t = du - np.einsum("fd,fd->f", du, fv.nb_h)[:, None] * fv.nb_h
+        f_p = pb[:, None] * fv.Sb_h
+        f_v = (self.case.viscosity * fv.alpha_b_h)[:, None] * du_t
+        out = {}
+        for z in self.mesh.boundary_zones():
+            m = (fv.bzone == z.id) & (self.bkind_h == WALL)
+            if m.any():
+                fp, fvis = f_p[m].sum(axis=0), f_v[m].sum(axis=0)
+                out[z.name] = {"pressure": fp.tolist(), "viscous": fvis.tolist(), "total": (fp + fvis).tolist()}
+        return out
+
+    def heat_flow(self) -> dict:
+        """Conduction + enthalpy flow into the domain per boundary zone (``simple.py:1003-1031``)."""
+        if not self.energy:
+            return {}
+        c, fv = self.case, self.fv
+        T = self.T.cpu().numpy()
+        Fb = self.Fb.cpu().numpy()
+        TP = T[fv.Pb_h]
+        tk, tv, kind = self.tkind_h, self.tval_h, self.bkind_h
+        Tb = np.where(tk == T_FIXED, tv, TP)
+        Tb = np.where((kind == OUTLET) & (Fb < 0), tv, Tb)
+        q = c.conductivity * fv.alpha_b_h * (Tb - TP) * (tk == T_FIXED)
+        q = q + np.where(tk == T_FLUX, tv * fv.Sb_mag_h, 0.0) - c.cp * Fb * Tb
+        return {z.name: float(q[fv.bzone == z.id].sum()) for z in self.mesh.boundary_zones()}
+
+    def mass_balance(self) -> dict:
+        Fb = self.Fb.cpu().numpy()
+        return {z.name: float(Fb[self.fv.bzone == z.id].sum()) for z in self.mesh.boundary_zones()
+                if (self.fv.bzone == z.id).any()}
+
+    def describe(self) -> dict:
+        return {"solver": f"pressure-based {self.algo}", "backend": "hip", "device": str(self.device),
+                "precision": "float64", "cells": self.nc, "faces": self.mesh.n_faces, "dim": self.dim,
+                "convection": self.case.scheme, "relax_p": self.relax_p,
+                "stability_rollbacks": list(self.guard_events),
+                "stability_guard_exhausted_at": self.guard_exhausted_at,
+                "pressure_solver": "AMG-PCG (SA-AMG, V(2,2) Jacobi, HIP)",
+                "max_non_orthogonality_deg": round(self.fv.non_orthogonality_deg, 2),
+                "setup_seconds": round(self.setup_seconds, 3)}
+
+
+__all__ = ["SimpleSolver", "auto_relax_p", "KrylovInfo"]

@@ -1,0 +1,216 @@
+/* libftmi — MI355X (gfx950) finite-volume SIMPLE/SIMPLEC kernels: the C ABI.
+ *
+ * Conventions (changed from the reference's flowtrace/native/fvm_kernels.h:1-11 as noted):
+ *   - every array argument is a DEVICE pointer (HIP global memory); every call takes a
+ *     trailing `void* stream` (a hipStream_t; NULL = the default stream) and is asynchronous
+ *     on it, except the Krylov solvers, which return their iteration counts and therefore
+ *     wait for their own control scalars;
+ *   - floating-point arrays are float64, C-contiguous, row-major (n, dim) unless a comment says
+ *     otherwise; every index array is int32 (the reference uses int64 for P, N, Pb);
+ *   - the caller owns and allocates every buffer (work buffers included); the library never
+ *     allocates device memory;
+ *   - every function returns an int status: 0 = success, < 0 = failure (text in
+ *     ftmi_last_error()), except where a return value is documented (Krylov: 1 converged,
+ *     0 not converged, < 0 failure);
+ *   - no function is re-entrant on a shared work buffer; one stream per solver.
+ *
+ * Face adjacency used by the fused SIMPLEC stages: for cell c, entries cptr[c]..cptr[c+1]-1
+ * list its interior faces in increasing face order, cface[k] = (f << 1) | (c is f's
+ * neighbour), cnbr[k] = the cell across face f; bptr/bface list its boundary faces.
+ * A cell-cell matrix in this layout ("LDU" form) is diag[nc] plus off[k], the coefficient
+ * of cnbr[k] in row c.
+ */
+#ifndef FTMI_H
+#define FTMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FTMI_ABI_VERSION 100
+
+/* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
+int ftmi_abi_version(void);
+/* new: text of the last failure on the calling thread */
+const char* ftmi_last_error(void);
+/* replaces ft_max_threads (fvm_kernels.h:28): HIP devices visible */
+int ftmi_device_count(void);
+
+/* ---------------------------------------------------------------- drop-in operators */
+
+/* replaces ft_csr_spmm (fvm_kernels.h:31): Y[nrows, k] = A @ X[ncols, k] */
+int ftmi_csr_spmm(int64_t nrows, int64_t k, const int32_t* indptr, const int32_t* indices, const double* data,
+                  const double* X, double* Y, void* stream);
+
+/* replaces ft_face_interp (fvm_kernels.h:35): out[f, :] = w[f] phi[P[f], :] + (1 - w[f]) phi[N[f], :] */
+int ftmi_face_interp(int64_t nf, int64_t k, const double* w, const int32_t* P, const int32_t* N, const double* phi,
+                     double* out, void* stream);
+
+/* replaces ft_ls_gradient (fvm_kernels.h:39): least-squares gradient over the Cdiff (iptr, iidx)
+ * and Cb (bptr, bidx) adjacencies, out[c, :dim] = Ginv[c] r[c] */
+int ftmi_ls_gradient(int64_t nc, int64_t dim, const int32_t* iptr, const int32_t* iidx, const int32_t* bptr,
+                     const int32_t* bidx, const int32_t* P, const int32_t* N, const double* wl, const double* d,
+                     const double* wlb, const double* db, const double* Ginv, const double* phi, const double* phib,
+                     double* out, void* stream);
+
+/* replaces ft_gg_gradient (fvm_kernels.h:48): Green-Gauss gradient with optional skewness
+ * correction grad_f . dxm_f (grad may be NULL) */
+int ftmi_gg_gradient(int64_t nc, int64_t dim, const int32_t* iptr, const int32_t* iidx, const double* isgn,
+                     const int32_t* bptr, const int32_t* bidx, const int32_t* P, const int32_t* N, const double* w,
+                     const double* S, const double* Sb, const double* vol, const double* dxm, const double* phi,
+                     const double* phib, const double* grad, double* out, void* stream);
+
+/* replaces ft_assemble (fvm_kernels.h:56): CSR data from diag and the face coefficients.
+ * The reference scatters through slot_PN/slot_NP in one sequential face loop; here each row
+ * is filled by one thread from a per-row list built once by ftmi_assemble_plan_host from the
+ * same slots: src[src_ptr[c]..src_ptr[c+1]) holds pairs (face << 1 | which, slot), which 0 =
+ * aPN, 1 = aNP, in face order. */
+int ftmi_assemble(int64_t nnz, int64_t nc, int64_t nf, const int32_t* indptr, const int32_t* slot_diag,
+                  const int32_t* src_ptr, const int32_t* src, const double* diag, const double* aPN,
+                  const double* aNP, double* data, void* stream);
+/* host helper (host pointers): builds src_ptr[nc+1] and src[4 nf] from the slots */
+int ftmi_assemble_plan_host(int64_t nc, int64_t nf, const int32_t* indptr, const int32_t* slot_PN,
+                            const int32_t* slot_NP, int32_t* src_ptr, int32_t* src);
+
+/* replaces ft_bicgstab_jacobi (fvm_kernels.h:64): Jacobi-right-preconditioned BiCGSTAB with the
+ * stopping rules of flowtrace/linalg/krylov.py:68-121. x: x0 in, solution out. work: device
+ * buffer of ftmi_bicgstab_work_bytes(n) bytes (the reference: 7 n doubles). *iters, *res are
+ * host pointers. Returns 1 converged, 0 not, < 0 error; *res = NaN on breakdown. */
+int64_t ftmi_bicgstab_work_bytes(int64_t n);
+int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                         const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
+                         int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
+
+/* ---------------------------------------------------------------- Krylov on the LDU form */
+
+/* new (the reference's momentum/energy solve, simple.py:489-497, on the face layout) */
+int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
+                      const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
+                      int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
+
+/* new: PCG of flowtrace/linalg/krylov.py:25-65 from x0 = 0 (x is output only), preconditioned
+ * by one V-cycle of the hierarchy `amg` (NULL: none); project != 0 removes the mean of the
+ * residual and of every preconditioned residual (the singular pressure system,
+ * simple.py:629-634). work: ftmi_pcg_work_bytes(n) bytes. */
+int64_t ftmi_pcg_work_bytes(int64_t n);
+int ftmi_pcg_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
+                 const double* b, double* x, double tol, int64_t maxiter, void* amg, int project, void* work,
+                 int64_t* iters, double* res, void* stream);
+int ftmi_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
+                 double* x, double tol, int64_t maxiter, void* amg, int project, void* work, int64_t* iters,
+                 double* res, void* stream);
+
+/* ---------------------------------------------------------------- smoothed-aggregation AMG
+ * replaces SmoothedAggregationAMG (flowtrace/linalg/amg.py:80-146). The hierarchy handle
+ * holds device pointers owned by the caller; levels are added finest first; the coarsest
+ * system is solved with a dense inverse (row-major). Level vectors b, x, y, r are caller-
+ * owned device buffers of the level's size. */
+void* ftmi_amg_create(double omega, int sweeps);
+void ftmi_amg_destroy(void* amg);
+int ftmi_amg_add_level(void* amg, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const double* a_val,
+                       const int32_t* p_ptr, const int32_t* p_col, const double* p_val, const int32_t* r_ptr,
+                       const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
+                       double* vy, double* vr);
+int ftmi_amg_set_coarse(void* amg, int64_t n, const double* inv, double* vb, double* vx);
+int ftmi_amg_vcycle(void* amg, const double* b, double* x, void* stream);
+/* host helper (host pointers): three-pass greedy aggregation of amg.py:26-57 over a strength
+ * graph in CSR form; returns the number of aggregates */
+int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* idx, int64_t* agg);
+
+/* ---------------------------------------------------------------- fused SIMPLEC stages */
+
+typedef struct ftmi_mesh {
+    int64_t nc, nf, nb, dim;
+    /* interior faces */
+    const int32_t *P, *N;
+    const double *w, *alpha, *wl;                       /* (nf) */
+    const double *S, *d, *Tvec, *dP_f, *dN_f, *dxm;     /* (nf, dim) */
+    /* boundary faces */
+    const int32_t* Pb;
+    const double *Sb, *db, *db_t, *nb_unit;             /* (nb, dim) */
+    const double *alpha_b, *wlb, *Sb_mag;               /* (nb) */
+    /* cells */
+    const double *vol, *Ginv;                           /* (nc), (nc, dim, dim) */
+    /* adjacency (see the file comment) */
+    const int32_t *cptr, *cface, *cnbr, *bptr, *bface;
+} ftmi_mesh;
+
+typedef struct ftmi_bc {
+    const int32_t* kind;      /* (nb): 0 wall, 1 inlet, 2 outlet, 3 pressure, 4 symmetry */
+    const int32_t* tkind;     /* (nb): 0 adiabatic, 1 fixed T, 2 heat flux */
+    const double* ub_fixed;   /* (nb, dim) */
+    const double* pb_fixed;   /* (nb) */
+    const double* tval;       /* (nb) */
+} ftmi_bc;
+
+/* reduction scratch: partials of >= 2048 * 8 doubles and a zero-initialised counter */
+typedef struct ftmi_red {
+    double* partials;
+    unsigned* counter;
+} ftmi_red;
+
+typedef struct ftmi_assemble_args {
+    const double *F, *Fb;
+    double gamma;
+    int energy;               /* 0 momentum (dim fields), 1 energy (one field) */
+    int nfields;
+    const double* phi[3];     /* (nc) each */
+    const double* grad[3];    /* (nc, dim) each */
+    const double* phib[3];    /* (nb) each */
+    const double* gp;         /* momentum: pressure gradient (nc, dim) */
+    double drive[3];
+    int has_drive;
+    double relax;
+    int second_order;
+    int simplec;              /* 1 SIMPLEC, 0 SIMPLE, -1 no dP */
+    double cp;
+    double *diag, *diag_r, *off, *b[3], *dP;
+    double* resid;            /* device, 2 nfields doubles */
+    double* inv_r;            /* 1 / diag_r (may be NULL) */
+    double* x0[3];            /* copies of phi[q], the Krylov start vectors (may be NULL) */
+} ftmi_assemble_args;
+
+/* new: boundary velocities and pressures (simple.py:305-325); U is (dim, nc) component-major,
+ * gU is (dim, nc, dim), ub is (dim, nb); gU/gp may be NULL (no lagged reconstruction) */
+int ftmi_simplec_bvals(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* U, const double* p, const double* gU,
+                       const double* gp, double* ub, double* pb, void* stream);
+/* new: boundary temperatures (simple.py:783-791) */
+int ftmi_energy_bvals(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* T, const double* gT, const double* Fb,
+                      double* Tb, void* stream);
+/* new: up to four least-squares gradients in one pass (fvmesh.py:268-286 / fvm_kernels.cpp:81-105);
+ * phi/phib/out are host arrays of device pointers; phib == NULL or phib[q] == NULL means the
+ * pressure-correction boundary value (1 - m_pres) phi_P with m_pres from bkind */
+int ftmi_ls_gradients(const ftmi_mesh* mesh, int nfields, const double* const* phi, const double* const* phib,
+                      double* const* out, const int32_t* bkind, void* stream);
+/* new: Green-Gauss gradient on the face adjacency (fvmesh.py:288-309) */
+int ftmi_gg_gradient_fused(const ftmi_mesh* mesh, const double* phi, const double* phib, const double* grad,
+                           double* out, void* stream);
+/* new: coefficients + right-hand sides + residual sums + relaxed system + SIMPLEC dP
+ * (simple.py:340-387, 526-590, 778-809) */
+int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi_assemble_args* args,
+                            const ftmi_red* red, void* stream);
+/* new: Rhie-Chow face fluxes, pressure-correction system, mass imbalance (simple.py:590-621);
+ * cont_out: device scalar sum |mass| */
+int ftmi_rhie_chow(const ftmi_mesh* mesh, const ftmi_bc* bc, double rho, const double* Ustar, const double* p,
+                   const double* gp, const double* dP, double* Fst, double* cf, double* Fbst, double* cb,
+                   double* diag_p, double* offp, double* mass, double* neg_mass, double* cont_out,
+                   const ftmi_red* red, void* stream);
+/* new: flux, pressure and velocity corrections (simple.py:639-646); sums_out (device, 3):
+ * sum p_new (before mean removal), sum |F|, sum |Fb| */
+int ftmi_correct(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* pp, const double* Ustar, const double* dP,
+                 const double* Fst, const double* cf, const double* Fbst, const double* cb, double relax_p,
+                 int remove_mean, double* U, double* p, double* F, double* Fb, double* gpp_out, double* sums_out,
+                 const ftmi_red* red, void* stream);
+/* new: initial face fluxes from the cell velocity (simple.py:110-112) */
+int ftmi_init_flux(const ftmi_mesh* mesh, const ftmi_bc* bc, double rho, const double* U, const double* ub,
+                   double* F, double* Fb, void* stream);
+/* new: *out = sum |x| (device scalar) */
+int ftmi_abs_sum(int64_t n, const double* x, double* out, const ftmi_red* red, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
