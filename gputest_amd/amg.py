@@ -1,13 +1,18 @@
 """Smoothed-aggregation AMG for the pressure-correction system (``flowtrace/linalg/amg.py``).
 
-Set-up follows ``amg.py:80-117``: symmetric strength ``|a_ij| >= theta sqrt(a_ii a_jj)``,
-three-pass greedy aggregation (``ftmi_amg_aggregate_host``), column-normalised tentative
-prolongator ``T``, ``rho(D^-1 A)`` by 15 power iterations from ``default_rng(0)``, smoothed
-prolongator ``P = T - 4/(3 rho) D^-1 A T``, Galerkin ``A_c = P^T A P``, until at most
-``max_coarse`` unknowns remain, then a dense inverse (shifted by ``1e-10 mean|diag|`` when the
-system is singular). The hierarchy is built on the host, like the reference's, and uploaded
-to HBM; the V(2,2) weighted-Jacobi cycles run as HIP kernels (``ftmi_amg_vcycle``, and inside
-``ftmi_pcg_ldu``).
+The algorithm is the reference's (``amg.py:80-146``): symmetric strength
+``|a_ij| >= theta sqrt(|a_ii a_jj|)`` with theta = 0.08, three-pass greedy aggregation,
+column-normalised tentative prolongator ``T``, ``rho(D^-1 A)`` from 15 power iterations
+started at ``default_rng(0).standard_normal(n)``, smoothed prolongator
+``P = T - 4/(3 rho) D^-1 A T``, Galerkin ``A_c = P^T A P``, levels until at most
+``max_coarse`` unknowns (or until aggregation stops coarsening), and a dense inverse of the
+coarsest matrix (shifted by ``1e-10 mean|diag|`` when the system is singular); V(2,2)
+weighted-Jacobi cycles with omega = 0.8.
+
+Set-up runs on the GPU (``ftmi_amgs_*``) except the greedy aggregation, which is
+sequential by definition (its result depends on the node order) and runs on the host over
+the strength mask (one byte per matrix entry copied back per level). The coarsest dense
+inverse (at most ``max_coarse`` squared entries) is computed on the host.
 """
 from __future__ import annotations
 
@@ -17,8 +22,44 @@ import torch
 
 from . import _native as nat
 
+_RNG_CACHE = {}
+
+
+def start_vector(n: int) -> np.ndarray:
+    """The power iteration's start vector, ``default_rng(0).standard_normal(n)``."""
+    v = _RNG_CACHE.get(n)
+    if v is None:
+        v = np.random.default_rng(0).standard_normal(n)
+        if len(_RNG_CACHE) < 64:
+            _RNG_CACHE[n] = v
+    return v
+
+
+class DevCSR:
+    """A CSR matrix in HBM (int32 pattern, float64 values)."""
+
+    def __init__(self, nrows, ncols, ptr, col, val):
+        self.n, self.ncols, self.ptr, self.col, self.val = int(nrows), int(ncols), ptr, col, val
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.numel())
+
+    @classmethod
+    def from_scipy(cls, A, device):
+        A = sp.csr_matrix(A, dtype=np.float64)
+        A.sort_indices()
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int32), device=device)  # noqa: E731
+        return cls(A.shape[0], A.shape[1], i32(A.indptr), i32(A.indices),
+                   torch.as_tensor(np.ascontiguousarray(A.data), device=device))
+
+    def to_scipy(self) -> sp.csr_matrix:
+        return sp.csr_matrix((self.val.cpu().numpy(), self.col.cpu().numpy(), self.ptr.cpu().numpy()),
+                             shape=(self.n, self.ncols))
+
 
 def strength(A: sp.csr_matrix, theta: float) -> sp.csr_matrix:
+    """Host form of the strength graph (tests and diagnostics)."""
     C = A.tocoo()
     dg = np.abs(A.diagonal())
     keep = (C.row != C.col) & (np.abs(C.data) >= theta * np.sqrt(dg[C.row] * dg[C.col]))
@@ -28,6 +69,7 @@ def strength(A: sp.csr_matrix, theta: float) -> sp.csr_matrix:
 
 
 def aggregate(S: sp.csr_matrix):
+    """Greedy aggregation of a host strength graph (``ftmi_amg_aggregate_host``)."""
     n = S.shape[0]
     ptr = np.ascontiguousarray(S.indptr, np.int64)
     idx = np.ascontiguousarray(S.indices, np.int64)
@@ -36,98 +78,163 @@ def aggregate(S: sp.csr_matrix):
     return agg, int(na)
 
 
-_RNG_CACHE = {}
-
-
-def _start_vector(n: int) -> np.ndarray:
-    v = _RNG_CACHE.get(n)
-    if v is None:
-        v = np.random.default_rng(0).standard_normal(n)
-        if len(_RNG_CACHE) < 64:
-            _RNG_CACHE[n] = v
-    return v
-
-
-def spectral_radius(M: sp.csr_matrix, iters: int = 15) -> float:
-    x = _start_vector(M.shape[0]).copy()
-    lam = 1.0
-    for _ in range(iters):
-        y = M @ x
-        lam = np.linalg.norm(y) / max(np.linalg.norm(x), 1e-300)
-        x = y / max(np.linalg.norm(y), 1e-300)
-    return lam
-
-
 class SmoothedAggregationAMG:
-    """A device-resident SA-AMG hierarchy; ``amg(b)`` applies one V-cycle on the GPU."""
+    """A device-resident SA-AMG hierarchy; ``amg(b)`` applies one V-cycle on the GPU.
 
-    def __init__(self, A: sp.csr_matrix, device="cuda", theta=0.08, max_coarse=300, max_levels=20, sweeps=2,
-                 omega=0.8, singular=False):
+    ``A`` is a host SciPy matrix or a ``DevCSR``; ``from_ldu`` builds the hierarchy of a
+    pressure matrix held in the face (LDU) form of an ``FVMesh``."""
+
+    debug = False
+
+    def __init__(self, A, device="cuda", theta=0.08, max_coarse=300, max_levels=20, sweeps=2, omega=0.8,
+                 singular=False, host_pattern=None):
         self.device = torch.device(device)
-        self.omega, self.sweeps = omega, sweeps
-        A = sp.csr_matrix(A, dtype=np.float64)
-        host = []
-        while A.shape[0] > max_coarse and len(host) < max_levels - 1:
-            S = strength(A, theta)
-            agg, na = aggregate(S)
-            if na >= A.shape[0] * 0.9:
+        self.theta, self.max_coarse, self.max_levels = theta, max_coarse, max_levels
+        self.omega, self.sweeps, self.singular = omega, sweeps, singular
+        A0 = A if isinstance(A, DevCSR) else DevCSR.from_scipy(A, self.device)
+        if host_pattern is None and not isinstance(A, DevCSR):
+            Ah = sp.csr_matrix(A)
+            Ah.sort_indices()
+            host_pattern = (np.ascontiguousarray(Ah.indptr, np.int32), np.ascontiguousarray(Ah.indices, np.int32))
+        self._setup(A0, host_pattern)
+
+    @classmethod
+    def from_ldu(cls, fv, diag, off, singular=False, **kw):
+        L = nat.lib()
+        data = torch.empty(fv.nnz, dtype=torch.float64, device=fv.device)
+        nat.check(L.ftmi_amgs_ldu_to_csr(fv.nc, nat.ptr(fv.cptr), nat.ptr(fv.he_slot), nat.ptr(fv.slot_diag),
+                                         nat.ptr(fv.indptr), nat.ptr(diag), nat.ptr(off), nat.ptr(data),
+                                         nat.stream_ptr()), "ldu_to_csr")
+        A0 = DevCSR(fv.nc, fv.nc, fv.indptr, fv.indices, data)
+        return cls(A0, fv.device, singular=singular, host_pattern=(fv.indptr_h, fv.indices_h), **kw)
+
+    # ------------------------------------------------------------------ set-up
+    def _spgemm(self, X: DevCSR, Y: DevCSR, xscale=None, agg=None, tval=None, omega_num=None, omega_scale=0.0):
+        L, st, dev = nat.lib(), nat.stream_ptr(), self.device
+        n = X.n
+        cnt = torch.zeros(n, dtype=torch.int32, device=dev)
+        args = (n, nat.ptr(X.ptr), nat.ptr(X.col), nat.ptr(X.val), nat.ptr(Y.ptr), nat.ptr(Y.col), nat.ptr(Y.val),
+                nat.ptr(xscale), nat.ptr(agg), nat.ptr(tval), nat.ptr(omega_num), float(omega_scale))
+        nat.check(L.ftmi_amgs_spgemm(0, *args, nat.ptr(cnt), None, None, None, nat.ptr(self._ovf), st), "spgemm")
+        ptr = self._scan(cnt)
+        nnz = int(ptr[n].item())
+        col = torch.empty(nnz, dtype=torch.int32, device=dev)
+        val = torch.empty(nnz, dtype=torch.float64, device=dev)
+        nat.check(L.ftmi_amgs_spgemm(1, *args, nat.ptr(cnt), nat.ptr(ptr), nat.ptr(col), nat.ptr(val),
+                                     nat.ptr(self._ovf), st), "spgemm")
+        if self.debug:
+            o = self._ovf.cpu().numpy()
+            print(f"[amg] spgemm rows {n} nnz {nnz} overflow {o.tolist()}", flush=True)
+        return DevCSR(n, Y.ncols, ptr, col, val)
+
+    def _scan(self, cnt):
+        n = cnt.numel()
+        out = torch.empty(n + 1, dtype=torch.int32, device=self.device)
+        nat.check(nat.lib().ftmi_scan_i32(n, nat.ptr(cnt), nat.ptr(out), nat.ptr(self._bsum), nat.stream_ptr()), "scan")
+        return out
+
+    def _transpose(self, P: DevCSR) -> DevCSR:
+        L, st, dev = nat.lib(), nat.stream_ptr(), self.device
+        na = P.ncols
+        cnt = torch.zeros(na, dtype=torch.int32, device=dev)
+        nat.check(L.ftmi_amgs_count_cols(P.nnz, nat.ptr(P.col), nat.ptr(cnt), st), "count_cols")
+        rptr = self._scan(cnt)
+        fill = torch.zeros(na, dtype=torch.int32, device=dev)
+        rcol = torch.empty(P.nnz, dtype=torch.int32, device=dev)
+        rval = torch.empty(P.nnz, dtype=torch.float64, device=dev)
+        nat.check(L.ftmi_amgs_transpose_fill(P.n, na, nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(P.val), nat.ptr(rptr),
+                                             nat.ptr(fill), nat.ptr(rcol), nat.ptr(rval), st), "transpose")
+        return DevCSR(na, P.n, rptr, rcol, rval)
+
+    def _setup(self, A: DevCSR, host_pattern):
+        L, st, dev = nat.lib(), nat.stream_ptr(), self.device
+        f64 = lambda *s: torch.empty(*s, dtype=torch.float64, device=dev)  # noqa: E731
+        self._ovf = torch.zeros(4, dtype=torch.int32, device=dev)
+        self._bsum = torch.empty(2048, dtype=torch.int64, device=dev)
+        partials = f64(2048 * 8)
+        self.levels, self.aggregates, self.rhos = [], [], []
+        pattern = host_pattern
+        while A.n > self.max_coarse and len(self.levels) < self.max_levels - 1:
+            n = A.n
+            diag, dinv = f64(n), f64(n)
+            nat.check(L.ftmi_amgs_diag(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag), nat.ptr(dinv),
+                                       st), "diag")
+            mask = torch.empty(A.nnz, dtype=torch.uint8, device=dev)
+            nat.check(L.ftmi_amgs_strength(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
+                                           float(self.theta), nat.ptr(mask), st), "strength")
+            if pattern is None:
+                pattern = (A.ptr.cpu().numpy(), A.col.cpu().numpy())
+            ptr_h, col_h = pattern
+            mask_h = mask.cpu().numpy()
+            agg_h = np.empty(n, np.int32)
+            size_h = np.empty(n, np.int32)
+            na = int(L.ftmi_amg_aggregate_mask_host(n, ptr_h.ctypes.data, col_h.ctypes.data, mask_h.ctypes.data,
+                                                    agg_h.ctypes.data, size_h.ctypes.data))
+            if na >= n * 0.9:
                 break
-            n = A.shape[0]
-            T = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, na))
-            cnt = np.asarray(T.sum(axis=0)).ravel()
-            T = T @ sp.diags(1.0 / np.sqrt(cnt))
-            Dinv = 1.0 / A.diagonal()
-            DA = sp.diags(Dinv) @ A
-            rho = spectral_radius(DA)
-            P = (T - (4.0 / 3.0 / rho) * (DA @ T)).tocsr()
-            host.append((A, P, Dinv, agg))
-            A = (P.T @ A @ P).tocsr()
-        self.coarse_n = A.shape[0]
-        Ad = A.toarray()
-        if singular:
+            agg = torch.as_tensor(agg_h, device=dev)
+            size = torch.as_tensor(size_h[:na].copy(), device=dev)
+            t_ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+            t_val = f64(n)
+            nat.check(L.ftmi_amgs_tentative(n, nat.ptr(agg), nat.ptr(size), nat.ptr(t_ptr), nat.ptr(t_val), st),
+                      "tentative")
+            T = DevCSR(n, na, t_ptr, agg, t_val)
+            x = torch.as_tensor(start_vector(n), device=dev)
+            y, rho = f64(n), f64(2)
+            nat.check(L.ftmi_amgs_power(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv), nat.ptr(x),
+                                        nat.ptr(y), 15, nat.ptr(partials), nat.ptr(rho), st), "power")
+            P = self._spgemm(A, T, xscale=dinv, agg=agg, tval=t_val, omega_num=rho, omega_scale=4.0 / 3.0)
+            R = self._transpose(P)
+            Ac = self._spgemm(R, self._spgemm(A, P))
+            self.levels.append((A, P, R, dinv))
+            self.aggregates.append(agg_h.astype(np.int64))
+            self.rhos.append(rho)
+            A, pattern = Ac, None
+        ovf = self._ovf.cpu().numpy()
+        if ovf[0]:
+            raise RuntimeError(f"AMG set-up: {ovf[0]} sparse-product rows exceeded the LDS hash capacity "
+                               f"(max distinct {ovf[1]}, max candidates {ovf[2]}, e.g. row {ovf[3]})")
+        self.coarse = A
+        Ad = A.to_scipy().toarray()
+        if self.singular:
             Ad = Ad + 1e-10 * np.abs(np.diag(Ad)).mean() * np.eye(Ad.shape[0])
         try:
             cinv = np.linalg.inv(Ad)
         except np.linalg.LinAlgError:
             cinv = np.linalg.pinv(Ad)
-        self.aggregates = [h[3] for h in host]
-        self.host_levels = [(h[0], h[1], h[2]) for h in host]
+        self.coarse_n = Ad.shape[0]
         self.coarse_inv_h = cinv
-        self._upload(host, cinv)
+        self._upload(cinv)
 
-    def _upload(self, host, cinv):
-        dev = self.device
-        f64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64), device=dev)  # noqa: E731
-        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int32), device=dev)  # noqa: E731
-        L = nat.lib()
-        self._keep = []
+    def _upload(self, cinv):
+        dev, L = self.device, nat.lib()
         self.handle = L.ftmi_amg_create(float(self.omega), int(self.sweeps))
-        for A, P, Dinv, _ in host:
-            P.sort_indices()
-            R = P.T.tocsr()
-            R.sort_indices()
-            n = A.shape[0]
-            t = {"a": (i32(A.indptr), i32(A.indices), f64(A.data)), "p": (i32(P.indptr), i32(P.indices), f64(P.data)),
-                 "r": (i32(R.indptr), i32(R.indices), f64(R.data)), "dinv": f64(Dinv),
-                 "v": [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(4)]}
-            self._keep.append(t)
-            nat.check(L.ftmi_amg_add_level(self.handle, n, *[nat.ptr(x) for x in t["a"]], *[nat.ptr(x) for x in t["p"]],
-                                           *[nat.ptr(x) for x in t["r"]], nat.ptr(t["dinv"]),
-                                           *[nat.ptr(v) for v in t["v"]]), "amg_add_level")
+        self._vecs = []
+        for A, P, R, dinv in self.levels:
+            v = [torch.zeros(A.n, dtype=torch.float64, device=dev) for _ in range(4)]
+            self._vecs.append(v)
+            nat.check(L.ftmi_amg_add_level(self.handle, A.n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val),
+                                           nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(P.val), nat.ptr(R.ptr),
+                                           nat.ptr(R.col), nat.ptr(R.val), nat.ptr(dinv), *[nat.ptr(t) for t in v]),
+                      "amg_add_level")
         nc = cinv.shape[0]
-        c = {"inv": f64(cinv), "b": torch.zeros(nc, dtype=torch.float64, device=dev),
-             "x": torch.zeros(nc, dtype=torch.float64, device=dev)}
-        self._keep.append(c)
-        nat.check(L.ftmi_amg_set_coarse(self.handle, nc, nat.ptr(c["inv"]), nat.ptr(c["b"]), nat.ptr(c["x"])),
+        self._cinv = torch.as_tensor(np.ascontiguousarray(cinv), device=dev)
+        self._cb = torch.zeros(nc, dtype=torch.float64, device=dev)
+        self._cx = torch.zeros(nc, dtype=torch.float64, device=dev)
+        nat.check(L.ftmi_amg_set_coarse(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb), nat.ptr(self._cx)),
                   "amg_set_coarse")
 
+    # ------------------------------------------------------------------ use
     @property
     def n_levels(self) -> int:
-        return len(self.host_levels) + 1
+        return len(self.levels) + 1
+
+    def prolongator(self, k: int) -> sp.csr_matrix:
+        return self.levels[k][1].to_scipy()
 
     def operator_complexity(self) -> float:
-        nnz = sum(A.nnz for A, _, _ in self.host_levels) + self.coarse_n ** 2
-        return nnz / max(self.host_levels[0][0].nnz if self.host_levels else 1, 1)
+        nnz = sum(A.nnz for A, _, _, _ in self.levels) + self.coarse.nnz
+        return nnz / max(self.levels[0][0].nnz if self.levels else self.coarse.nnz, 1)
 
     def __call__(self, b: torch.Tensor) -> torch.Tensor:
         b = b.contiguous()

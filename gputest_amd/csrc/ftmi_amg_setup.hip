@@ -1,0 +1,463 @@
+// Smoothed-aggregation AMG set-up on the GPU (flowtrace/linalg/amg.py:60-117).
+//
+// Everything but the greedy aggregation runs here: CSR of the pressure matrix from its face
+// form, the diagonal, the strength mask |a_ij| >= theta sqrt(|a_ii a_jj|), the tentative
+// prolongator weights 1/sqrt(|aggregate|), rho(D^-1 A) by power iteration, the smoothed
+// prolongator P = T - 4/(3 rho) D^-1 A T, R = P^T, and the Galerkin product R (A P). The
+// aggregation itself (three greedy passes whose result depends on the node order) runs on
+// the host over the strength mask copied back (ftmi_amg_aggregate_mask_host).
+//
+// Sparse products are computed one wavefront per output row: the lanes insert the row's
+// candidate columns into an LDS hash table (integer compare-and-swap, so the set is
+// deterministic), the distinct columns are ranked into sorted order, and each column's value
+// is summed by one lane in the fixed order of the left operand's row. Results are therefore
+// independent of the schedule. Rows are sized in a count pass, the caller scans the counts
+// into row pointers and allocates, and a fill pass writes the entries.
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/ftmi.h"
+#include "ftmi_common.h"
+
+using namespace ftmi;
+
+namespace {
+
+constexpr int kHash = 2048;     // LDS hash slots per row (distinct columns must stay below)
+constexpr int kHashLimit = 1536;
+
+__global__ void __launch_bounds__(kBlock) k_ldu_to_csr(int64_t n, const int32_t* __restrict__ cptr,
+                                                       const int32_t* __restrict__ he_slot,
+                                                       const int32_t* __restrict__ slot_diag,
+                                                       const int32_t* __restrict__ indptr,
+                                                       const double* __restrict__ diag,
+                                                       const double* __restrict__ off, double* __restrict__ data) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t c = lo + threadIdx.x; c < hi; c += kBlock) {
+        for (int32_t j = indptr[c]; j < indptr[c + 1]; ++j) data[j] = 0.0;
+        data[slot_diag[c]] = diag[c];
+        for (int32_t k = cptr[c]; k < cptr[c + 1]; ++k) data[he_slot[k]] += off[k];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_diag(int64_t n, const int32_t* __restrict__ ptr,
+                                                 const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                 double* __restrict__ diag, double* __restrict__ dinv) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        double d = 0.0;
+        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j)
+            if (col[j] == i) d += val[j];
+        diag[i] = d;
+        dinv[i] = 1.0 / d;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_strength(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ diag, double theta,
+                                                     uint8_t* __restrict__ mask) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const double di = fabs(diag[i]);
+        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) {
+            const int32_t c = col[j];
+            mask[j] = (c != i && fabs(val[j]) >= theta * sqrt(di * fabs(diag[c]))) ? 1 : 0;
+        }
+    }
+}
+
+// T as a CSR matrix with one entry per row: (agg[i], 1/sqrt(size of aggregate agg[i]))
+__global__ void __launch_bounds__(kBlock) k_tentative(int64_t n, const int32_t* __restrict__ agg,
+                                                      const int32_t* __restrict__ agg_size,
+                                                      int32_t* __restrict__ t_ptr, double* __restrict__ t_val) {
+    int64_t lo, hi;
+    block_range(n + 1, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        t_ptr[i] = (int32_t)i;
+        if (i < n) t_val[i] = 1.0 / sqrt((double)agg_size[agg[i]]);
+    }
+}
+
+// ---------------------------------------------------------------- power iteration for rho(D^-1 A)
+__global__ void __launch_bounds__(kBlock) k_pow_spmv(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ dinv,
+                                                     const double* __restrict__ x, double* __restrict__ y,
+                                                     double* partials) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const double di = dinv[i];
+        double s = 0.0;
+        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) s += (di * val[j]) * x[col[j]];
+        y[i] = s;
+        acc[0] += s * s;
+        acc[1] += x[i] * x[i];
+    }
+    block_partials<2>(acc, partials);
+}
+
+// lam = ||y|| / max(||x||, 1e-300); scale[0] = max(||y||, 1e-300)
+__global__ void __launch_bounds__(kBlock) k_pow_fin(const double* partials, int nparts, double* lam, double* scale) {
+    double tot[2];
+    if (finalize_sum<2>(partials, nparts, tot)) {
+        const double ny = sqrt(tot[0]), nx = sqrt(tot[1]);
+        *lam = ny / (nx > 1e-300 ? nx : 1e-300);
+        *scale = ny > 1e-300 ? ny : 1e-300;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pow_scale(int64_t n, const double* __restrict__ y, double* __restrict__ x,
+                                                      const double* scale) {
+    const double s = *scale;
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) x[i] = y[i] / s;
+}
+
+// ---------------------------------------------------------------- wave-per-row sparse products
+struct CsrV {
+    const int32_t* ptr;
+    const int32_t* col;
+    const double* val;
+};
+
+// multiplicative hash into a table of 2^bits slots
+__device__ inline uint32_t hslot(int32_t c, int bits) { return ((uint32_t)c * 2654435761u) >> (32 - bits); }
+
+// position of column c in row k of Y (columns sorted), or -1
+__device__ inline int32_t find_in_row(const CsrV& Y, int32_t k, int32_t c) {
+    int32_t lo = Y.ptr[k], hi = Y.ptr[k + 1] - 1;
+    while (lo <= hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        const int32_t m = Y.col[mid];
+        if (m == c) return mid;
+        if (m < c)
+            lo = mid + 1;
+        else
+            hi = mid - 1;
+    }
+    return -1;
+}
+
+// C = Xs Y row by row, Xs = diag(xscale) X when xscale != nullptr. In "prolongator" mode
+// (agg != nullptr) the row is finished as C_rc = [c == agg[r]] t[r] - omega C_rc, the
+// smoothed prolongator T - omega D^-1 A T with Y = T.
+template <bool FILL>
+__global__ void __launch_bounds__(kWave) k_spgemm(int64_t nrows, CsrV X, CsrV Y, const double* __restrict__ xscale,
+                                                  const int32_t* __restrict__ agg, const double* __restrict__ tval,
+                                                  const double* omega_num, double omega_scale,
+                                                  int32_t* __restrict__ rowcnt, const int32_t* __restrict__ cptr,
+                                                  int32_t* __restrict__ ccol, double* __restrict__ cval,
+                                                  int* overflow) {
+    __shared__ int32_t keys[kHash];
+    __shared__ int32_t dense[kHash];
+    __shared__ int32_t nk, nd;
+    const int lane = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int32_t xb = X.ptr[r], xe = X.ptr[r + 1];
+        // table size: a power of two >= twice this row's candidate count (at least 64 slots)
+        int bound = 0;
+        for (int32_t j = xb + lane; j < xe; j += kWave) bound += Y.ptr[X.col[j] + 1] - Y.ptr[X.col[j]];
+        bound = wave_sum(bound);
+        int bits = 6;
+        while (bits < 11 && (1 << bits) < 2 * bound) ++bits;
+        const int hsize = 1 << bits;
+        for (int h = lane; h < hsize; h += kWave) keys[h] = -1;
+        if (lane == 0) {
+            nk = 0;
+            nd = 0;
+        }
+        __syncthreads();
+        for (int32_t j = xb + lane; j < xe; j += kWave) {
+            const int32_t k = X.col[j];
+            for (int32_t q = Y.ptr[k]; q < Y.ptr[k + 1]; ++q) {
+                const int32_t c = Y.col[q];
+                uint32_t h = hslot(c, bits);
+                for (int probe = 0; probe < hsize; ++probe) {
+                    const int32_t old = atomicCAS(&keys[h], -1, c);
+                    if (old == -1) {
+                        atomicAdd(&nk, 1);
+                        break;
+                    }
+                    if (old == c) break;
+                    h = (h + 1) & (hsize - 1);
+                }
+            }
+        }
+        __syncthreads();
+        const int n_keys = nk;
+        if (n_keys > kHashLimit || n_keys >= hsize) {
+            // overflow[0]: rows over capacity, [1]: largest distinct count, [2]: largest
+            // candidate bound, [3]: one offending row
+            if (lane == 0) {
+                atomicAdd(&overflow[0], 1);
+                atomicMax(&overflow[1], n_keys);
+                atomicMax(&overflow[2], bound);
+                overflow[3] = (int)r;
+            }
+            __syncthreads();
+            continue;
+        }
+        if (!FILL) {
+            if (lane == 0) rowcnt[r] = n_keys;
+            __syncthreads();
+            continue;
+        }
+        // compact the distinct columns (any order), then rank them: keys[rank(c)] = c
+        for (int h = lane; h < hsize; h += kWave) {
+            const int32_t c = keys[h];
+            if (c >= 0) dense[atomicAdd(&nd, 1)] = c;
+        }
+        __syncthreads();
+        for (int t = lane; t < n_keys; t += kWave) {
+            const int32_t c = dense[t];
+            int rank = 0;
+            for (int u = 0; u < n_keys; ++u) rank += dense[u] < c ? 1 : 0;
+            keys[rank] = c;
+        }
+        __syncthreads();
+        const int32_t base = cptr[r];
+        const double xs = xscale ? xscale[r] : 1.0;
+        for (int t = lane; t < n_keys; t += kWave) {
+            const int32_t c = keys[t];
+            double v = 0.0;
+            for (int32_t j = xb; j < xe; ++j) {
+                const int32_t q = find_in_row(Y, X.col[j], c);
+                if (q >= 0) v += (xscale ? xs * X.val[j] : X.val[j]) * Y.val[q];
+            }
+            if (agg) {
+                const double om = omega_scale / *omega_num;  // (4/3) / rho
+                v = (c == agg[r] ? tval[r] : 0.0) - om * v;
+            }
+            ccol[base + t] = c;
+            cval[base + t] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- transpose (R = P^T)
+__global__ void __launch_bounds__(kBlock) k_count_cols(int64_t nnz, const int32_t* __restrict__ col,
+                                                       int32_t* __restrict__ cnt) {
+    int64_t lo, hi;
+    block_range(nnz, lo, hi);
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kBlock) atomicAdd(&cnt[col[j]], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_place(int64_t nrows, const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                  const int32_t* __restrict__ tptr, int32_t* __restrict__ fill,
+                                                  int32_t* __restrict__ tcol, double* __restrict__ tval) {
+    int64_t lo, hi;
+    block_range(nrows, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) {
+            const int32_t c = col[j];
+            const int32_t pos = tptr[c] + atomicAdd(&fill[c], 1);
+            tcol[pos] = (int32_t)i;
+            tval[pos] = val[j];
+        }
+    }
+}
+
+// sort each row's entries by column, one wavefront per row: the row is staged in LDS and every
+// entry is placed at its rank (columns within a row are distinct); rows longer than the stage
+// are sorted by one lane in place (insertion sort)
+constexpr int kSortCap = 1024;
+__global__ void __launch_bounds__(kWave) k_sort_rows(int64_t nrows, const int32_t* __restrict__ ptr,
+                                                     int32_t* __restrict__ col, double* __restrict__ val) {
+    __shared__ int32_t sc[kSortCap];
+    __shared__ double sv[kSortCap];
+    const int lane = threadIdx.x;
+    for (int64_t i = blockIdx.x; i < nrows; i += gridDim.x) {
+        const int32_t b = ptr[i], e = ptr[i + 1], L = e - b;
+        if (L <= 1) continue;
+        if (L <= kSortCap) {
+            for (int t = lane; t < L; t += kWave) {
+                sc[t] = col[b + t];
+                sv[t] = val[b + t];
+            }
+            __syncthreads();
+            for (int t = lane; t < L; t += kWave) {
+                const int32_t c = sc[t];
+                int rank = 0;
+                for (int u = 0; u < L; ++u) rank += sc[u] < c ? 1 : 0;
+                col[b + rank] = c;
+                val[b + rank] = sv[t];
+            }
+            __syncthreads();
+        } else if (lane == 0) {
+            for (int32_t j = b + 1; j < e; ++j) {
+                const int32_t c = col[j];
+                const double v = val[j];
+                int32_t q = j - 1;
+                while (q >= b && col[q] > c) {
+                    col[q + 1] = col[q];
+                    val[q + 1] = val[q];
+                    --q;
+                }
+                col[q + 1] = c;
+                val[q + 1] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- exclusive scan of int32 counts
+__global__ void __launch_bounds__(kBlock) k_scan_blocks(int64_t n, const int32_t* __restrict__ in,
+                                                        int64_t* __restrict__ bsum) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    long long acc[1] = {0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc[0] += in[i];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = acc[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(int64_t n, const int32_t* __restrict__ in,
+                                                       const int64_t* __restrict__ bsum, int32_t* __restrict__ out) {
+    __shared__ long long base;
+    __shared__ long long part[kBlock];
+    if (threadIdx.x == 0) {
+        // the XCD-permuted chunk order: chunks are laid out by logical index, so sum the
+        // block sums of every block whose logical index is smaller
+        const int g = gridDim.x, b = blockIdx.x;
+        auto logical_of = [g](int bb) { return (g >= kNXcd && g % kNXcd == 0) ? (bb % kNXcd) * (g / kNXcd) + bb / kNXcd : bb; };
+        const int me = logical_of(b);
+        long long s = 0;
+        for (int bb = 0; bb < g; ++bb)
+            if (logical_of(bb) < me) s += bsum[bb];
+        base = s;
+    }
+    __syncthreads();
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    long long run = base;
+    for (int64_t start = lo; start < hi; start += kBlock) {
+        const int64_t i = start + threadIdx.x;
+        const long long v = i < hi ? in[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        // Hillis-Steele inclusive scan in LDS
+        for (int off = 1; off < kBlock; off <<= 1) {
+            const long long t = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < hi) out[i] = (int32_t)(run + part[threadIdx.x] - v);
+        run += part[kBlock - 1];
+        __syncthreads();
+    }
+    if (hi == n && lo < hi && threadIdx.x == 0) out[n] = (int32_t)run;
+    if (n == 0 && blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftmi_amgs_ldu_to_csr(int64_t n, const int32_t* cptr, const int32_t* he_slot, const int32_t* slot_diag,
+                         const int32_t* indptr, const double* diag, const double* off, double* data, void* stream) {
+    k_ldu_to_csr<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, cptr, he_slot, slot_diag, indptr, diag, off,
+                                                                   data);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_ldu_to_csr");
+    return 0;
+}
+
+int ftmi_amgs_diag(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, double* diag, double* dinv,
+                   void* stream) {
+    k_diag<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, ptr, col, val, diag, dinv);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_diag");
+    return 0;
+}
+
+int ftmi_amgs_strength(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* diag,
+                       double theta, uint8_t* mask, void* stream) {
+    k_strength<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, ptr, col, val, diag, theta, mask);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_strength");
+    return 0;
+}
+
+int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, int32_t* t_ptr, double* t_val,
+                        void* stream) {
+    k_tentative<<<grid_for(n + 1), kBlock, 0, (hipStream_t)stream>>>(n, agg, agg_size, t_ptr, t_val);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_tentative");
+    return 0;
+}
+
+// rho(D^-1 A) by `iters` power iterations from x (overwritten); y: n doubles of work;
+// partials: >= 2048 * 2 doubles; out[0] = lam (device), out[1] used as scratch.
+int ftmi_amgs_power(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* dinv,
+                    double* x, double* y, int iters, double* partials, double* out, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int g = grid_for(n);
+    for (int it = 0; it < iters; ++it) {
+        k_pow_spmv<<<g, kBlock, 0, s>>>(n, ptr, col, val, dinv, x, y, partials);
+        k_pow_fin<<<1, kBlock, 0, s>>>(partials, g, &out[0], &out[1]);
+        k_pow_scale<<<g, kBlock, 0, s>>>(n, y, x, &out[1]);
+    }
+    FTMI_LAUNCH_CHECK("ftmi_amgs_power");
+    return 0;
+}
+
+// Row counts (fill == 0) or entries (fill == 1) of C = diag(xscale) X Y; with agg != NULL,
+// C = T - (omega_scale / *omega_num) diag(xscale) X Y with T = (agg, tval). overflow (4 int)
+// records rows whose distinct columns exceed the LDS hash capacity (count, max, bound, row).
+int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_t* x_col, const double* x_val,
+                     const int32_t* y_ptr, const int32_t* y_col, const double* y_val, const double* xscale,
+                     const int32_t* agg, const double* tval, const double* omega_num, double omega_scale,
+                     int32_t* rowcnt, const int32_t* c_ptr, int32_t* c_col, double* c_val, int* overflow,
+                     void* stream) {
+    if (nrows <= 0) return 0;
+    CsrV X{x_ptr, x_col, x_val}, Y{y_ptr, y_col, y_val};
+    int64_t g = nrows < 65536 ? nrows : 65536;
+    if (fill)
+        k_spgemm<true><<<(int)g, kWave, 0, (hipStream_t)stream>>>(nrows, X, Y, xscale, agg, tval, omega_num,
+                                                                   omega_scale, rowcnt, c_ptr, c_col, c_val,
+                                                                   overflow);
+    else
+        k_spgemm<false><<<(int)g, kWave, 0, (hipStream_t)stream>>>(nrows, X, Y, xscale, agg, tval, omega_num,
+                                                                    omega_scale, rowcnt, c_ptr, c_col, c_val,
+                                                                    overflow);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_spgemm");
+    return 0;
+}
+
+// out[0..n] = exclusive prefix sums of in[0..n-1] (out[n] = total); bsum: >= 2048 int64.
+int ftmi_scan_i32(int64_t n, const int32_t* in, int32_t* out, int64_t* bsum, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    const int g = grid_for(n > 0 ? n : 1);
+    k_scan_blocks<<<g, kBlock, 0, s>>>(n, in, bsum);
+    k_scan_apply<<<g, kBlock, 0, s>>>(n, in, bsum, out);
+    FTMI_LAUNCH_CHECK("ftmi_scan_i32");
+    return 0;
+}
+
+// R = P^T for P (nrows x ncols) in CSR; r_ptr must hold the exclusive scan of the column
+// counts (ftmi_amgs_count_cols + ftmi_scan_i32); fill: ncols int32 zeros (work).
+int ftmi_amgs_count_cols(int64_t nnz, const int32_t* col, int32_t* cnt, void* stream) {
+    if (nnz > 0) k_count_cols<<<grid_for(nnz), kBlock, 0, (hipStream_t)stream>>>(nnz, col, cnt);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_count_cols");
+    return 0;
+}
+
+int ftmi_amgs_transpose_fill(int64_t nrows, int64_t ncols, const int32_t* ptr, const int32_t* col, const double* val,
+                             const int32_t* r_ptr, int32_t* fill, int32_t* r_col, double* r_val, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    k_place<<<grid_for(nrows), kBlock, 0, s>>>(nrows, ptr, col, val, r_ptr, fill, r_col, r_val);
+    const int64_t gs = ncols < 65536 ? (ncols > 0 ? ncols : 1) : 65536;
+    k_sort_rows<<<(int)gs, kWave, 0, s>>>(ncols, r_ptr, r_col, r_val);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_transpose_fill");
+    return 0;
+}
+
+}  // extern "C"

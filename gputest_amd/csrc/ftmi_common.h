@@ -112,6 +112,29 @@ __device__ inline bool finalize_sum(const double* partials, int nparts, double (
     return threadIdx.x == 0;
 }
 
+// Consumer-side total computed by every workgroup of the consuming kernel (all threads get
+// it): the producer ran at most kRedGrid workgroups, so each consumer workgroup re-adds the
+// partials in the same fixed order and every workgroup sees bit-identical totals.
+constexpr int kRedGrid = 1024;
+inline int grid_red(int64_t n) {
+    const int g = grid_for(n);
+    return g < kRedGrid ? g : kRedGrid;
+}
+
+template <int M>
+__device__ inline void consume_sum(const double* partials, int nparts, double (&tot)[M]) {
+    __shared__ double bc[M];
+    double t[M];
+    if (finalize_sum<M>(partials, nparts, t)) {
+#pragma unroll
+        for (int j = 0; j < M; ++j) bc[j] = t[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < M; ++j) tot[j] = bc[j];
+    __syncthreads();
+}
+
 // Generic finalize: out[j] = scale * total[j].
 template <int M>
 __global__ void __launch_bounds__(kBlock) fin_store(const double* partials, int nparts, double* out, double scale) {

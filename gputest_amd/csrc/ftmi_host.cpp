@@ -71,4 +71,37 @@ int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* id
     return na;
 }
 
+// The same aggregation over a CSR pattern and a strength mask (mask[j] != 0: entry j is a
+// strong off-diagonal connection), int32 in and out; size[a] receives each aggregate's size.
+int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
+                                     int32_t* agg, int32_t* size) {
+    int64_t na = 0;
+    for (int64_t i = 0; i < n; ++i) agg[i] = -1;
+    for (int64_t i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        bool all_free = true;
+        for (int32_t k = ptr[i]; k < ptr[i + 1] && all_free; ++k)
+            if (mask[k]) all_free = agg[col[k]] < 0;
+        if (!all_free) continue;
+        agg[i] = (int32_t)na;
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
+            if (mask[k]) agg[col[k]] = (int32_t)na;
+        ++na;
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+            if (mask[k] && agg[col[k]] >= 0) {
+                agg[i] = agg[col[k]];
+                break;
+            }
+        }
+    }
+    for (int64_t i = 0; i < n; ++i)
+        if (agg[i] < 0) agg[i] = (int32_t)na++;
+    for (int64_t a = 0; a < na; ++a) size[a] = 0;
+    for (int64_t i = 0; i < n; ++i) ++size[agg[i]];
+    return na;
+}
+
 }  // extern "C"

@@ -32,7 +32,11 @@ using namespace ftmi;
 namespace {
 
 // ------------------------------------------------------------------ device state block
-enum { S_RHO, S_ALPHA, S_OMEGA, S_RNORM, S_BNORM, S_BABS, S_RHONEW, S_RES, S_RZ, S_BETA, S_ZMEAN, S_BMEAN, S_NS = 16 };
+// S_RZ, S_RZ + 1: r.z of odd / even PCG iterations (the kernel that computes the new value
+// reads the previous one, so they live in different slots).
+// Rule for every kernel below: workgroup 0 may write a state slot only if no workgroup of the
+// same kernel reads that slot (workgroups that start later would see the new value).
+enum { S_RHO, S_ALPHA, S_OMEGA, S_RNORM, S_BNORM, S_BABS, S_RHONEW, S_RES, S_RZ, S_RZ1, S_ZMEAN, S_BMEAN, S_NS = 16 };
 enum { I_DONE, I_CONV, I_ITERS, I_CUR, I_FIXUP, I_NI = 8 };
 
 struct State {  // 192 bytes, copied to the host as one block
@@ -74,11 +78,17 @@ static int post_state(Pinned* pn, int slot, const State* d, hipStream_t s) {
 static int wait_state(Pinned* pn, int slot) { return hipEventSynchronize(pn->ev[slot]) == hipSuccess ? 0 : -1; }
 
 // ------------------------------------------------------------------ BiCGSTAB kernels
+// One iteration is four launches: p-update fused into v = A phat (phat recomputed at the
+// neighbours from r, p, v of the previous iteration), s-update fused into t = A shat, the
+// residual/iterate update, and the state finalize that the host polls. p and v alternate
+// between two buffers so that no kernel reads what it writes. The scalars of a reduction are
+// computed by every workgroup of the kernel that consumes it (consume_sum), so only the
+// end-of-iteration state needs its own one-workgroup kernel.
 template <class Op>
 __global__ void __launch_bounds__(kBlock) bi_init(int64_t n, Op A, const double* __restrict__ b,
                                                   const double* __restrict__ x, double* __restrict__ r,
                                                   double* __restrict__ rh, double* __restrict__ p,
-                                                  double* __restrict__ v, State* S, RedBuf rb, double tol, int rel) {
+                                                  double* __restrict__ v, double* partials) {
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[2] = {0.0, 0.0};
@@ -91,7 +101,7 @@ __global__ void __launch_bounds__(kBlock) bi_init(int64_t n, Op A, const double*
         acc[0] += ri * ri;
         acc[1] += b[i] * b[i];
     }
-    block_partials<2>(acc, rb.partials);
+    block_partials<2>(acc, partials);
 }
 
 __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, int nparts, State* S, double tol,
@@ -119,118 +129,101 @@ __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, in
     }
 }
 
-__global__ void __launch_bounds__(kBlock) bi_p(int64_t n, const double* __restrict__ r, double* __restrict__ p,
-                                               const double* __restrict__ v, const double* __restrict__ inv,
-                                               double* __restrict__ ph, const State* S) {
+// p = r + beta (p_prev - omega v_prev), phat = inv p, v = A phat; partials of rhat . v
+template <class Op>
+__global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const double* __restrict__ r,
+                                                const double* __restrict__ p_prev, const double* __restrict__ v_prev,
+                                                const double* __restrict__ inv, const double* __restrict__ rh,
+                                                double* __restrict__ p, double* __restrict__ ph,
+                                                double* __restrict__ v, const State* S, double* part_out) {
     if (S->st[I_DONE]) return;
     const double* sc = S->sc;
     const double beta = (sc[S_RHONEW] / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]);
     const double om = sc[S_OMEGA];
-    int64_t lo, hi;
-    block_range(n, lo, hi);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double pi = r[i] + beta * (p[i] - om * v[i]);
-        p[i] = pi;
-        ph[i] = inv[i] * pi;
-    }
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kBlock) bi_v(int64_t n, Op A, const double* __restrict__ ph,
-                                               const double* __restrict__ rh, double* __restrict__ v, State* S,
-                                               RedBuf rb, int k) {
-    if (S->st[I_DONE]) return;
+    auto phat = [&](int64_t j) { return inv[j] * (r[j] + beta * (p_prev[j] - om * v_prev[j])); };
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[1] = {0.0};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double vi = A.row(i, ph);
+        const double pi = r[i] + beta * (p_prev[i] - om * v_prev[i]);
+        p[i] = pi;
+        ph[i] = inv[i] * pi;
+        const double vi = A.row_f(i, phat);
         v[i] = vi;
         acc[0] += rh[i] * vi;
     }
-    block_partials<1>(acc, rb.partials);
+    block_partials<1>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_bi_v(const double* partials, int nparts, State* S, int k) {
+// alpha = rho / (rhat . v); s = r - alpha v, shat = inv s, t = A shat; partials s.s, t.t, t.s
+template <class Op>
+__global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const double* __restrict__ r,
+                                                const double* __restrict__ v, const double* __restrict__ inv,
+                                                double* __restrict__ s, double* __restrict__ sh,
+                                                double* __restrict__ t, State* S, const double* part_in, int np_in,
+                                                double* part_out, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
-    if (finalize_sum<1>(partials, nparts, tot)) {
-        double* sc = S->sc;
-        sc[S_RHO] = sc[S_RHONEW];
-        if (tot[0] == 0.0) {
+    consume_sum<1>(part_in, np_in, tot);
+    const double rho = S->sc[S_RHONEW];
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (tot[0] == 0.0) {  // breakdown: the iteration stops before its first update
+        if (lead) {
+            S->sc[S_RHO] = rho;
             S->st[I_DONE] = 1;
             S->st[I_CONV] = 0;
             S->st[I_ITERS] = k;
-            sc[S_RES] = sc[S_RNORM] / sc[S_BNORM];
-        } else {
-            sc[S_ALPHA] = sc[S_RHO] / tot[0];
+            S->sc[S_RES] = S->sc[S_RNORM] / S->sc[S_BNORM];
         }
+        return;
     }
-}
-
-__global__ void __launch_bounds__(kBlock) bi_s(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
-                                               const double* __restrict__ inv, double* __restrict__ s,
-                                               double* __restrict__ sh, State* S, RedBuf rb, double tol, int k) {
-    if (S->st[I_DONE]) return;
-    const double alpha = S->sc[S_ALPHA];
+    const double alpha = rho / tot[0];
+    if (lead) {
+        S->sc[S_RHO] = rho;
+        S->sc[S_ALPHA] = alpha;
+    }
+    auto shat = [&](int64_t j) { return inv[j] * (r[j] - alpha * v[j]); };
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    double acc[3] = {0.0, 0.0, 0.0};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         const double si = r[i] - alpha * v[i];
         s[i] = si;
         sh[i] = inv[i] * si;
-        acc[0] += si * si;
-    }
-    block_partials<1>(acc, rb.partials);
-}
-
-__global__ void __launch_bounds__(kBlock) fin_bi_s(const double* partials, int nparts, State* S, double tol, int k) {
-    if (S->st[I_DONE]) return;
-    double tot[1];
-    if (finalize_sum<1>(partials, nparts, tot)) {
-        double* sc = S->sc;
-        const double snorm = sqrt(tot[0]);
-        if (snorm / sc[S_BNORM] < tol) {
-            S->st[I_DONE] = 1;
-            S->st[I_CONV] = 1;
-            S->st[I_ITERS] = k;
-            S->st[I_FIXUP] = 1;  // x = x_cur + alpha phat, applied after the loop
-            sc[S_RES] = snorm / sc[S_BNORM];
-        }
-    }
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kBlock) bi_t(int64_t n, Op A, const double* __restrict__ sh,
-                                               const double* __restrict__ s, double* __restrict__ t, State* S,
-                                               RedBuf rb) {
-    if (S->st[I_DONE]) return;
-    int64_t lo, hi;
-    block_range(n, lo, hi);
-    double acc[2] = {0.0, 0.0};
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double ti = A.row(i, sh);
+        const double ti = A.row_f(i, shat);
         t[i] = ti;
-        acc[0] += ti * ti;
-        acc[1] += ti * s[i];
+        acc[0] += si * si;
+        acc[1] += ti * ti;
+        acc[2] += ti * si;
     }
-    block_partials<2>(acc, rb.partials);
+    block_partials<3>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_bi_t(const double* partials, int nparts, State* S) {
-    if (S->st[I_DONE]) return;
-    double tot[2];
-    if (finalize_sum<2>(partials, nparts, tot)) S->sc[S_OMEGA] = tot[0] > 0.0 ? tot[1] / tot[0] : 0.0;
-}
-
+// early exit on ||s||; omega = (t.s)/(t.t); r = s - omega t; x_other = x_cur + alpha phat + omega shat
 __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const double* __restrict__ s, const double* __restrict__ t,
                                                const double* __restrict__ ph, const double* __restrict__ sh,
                                                const double* __restrict__ rh, double* __restrict__ r,
-                                               double* __restrict__ x0, double* __restrict__ x1, State* S, RedBuf rb,
-                                               double tol, int k, int maxiter) {
+                                               double* __restrict__ x0, double* __restrict__ x1, State* S,
+                                               const double* part_in, int np_in, double* part_out, double tol,
+                                               int k) {
     if (S->st[I_DONE]) return;
-    const double alpha = S->sc[S_ALPHA], om = S->sc[S_OMEGA];
+    double tot[3];
+    consume_sum<3>(part_in, np_in, tot);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const double snorm = sqrt(tot[0]), bnorm = S->sc[S_BNORM];
+    if (snorm / bnorm < tol) {  // x = x_cur + alpha phat, applied after the loop
+        if (lead) {
+            S->st[I_DONE] = 1;
+            S->st[I_CONV] = 1;
+            S->st[I_ITERS] = k;
+            S->st[I_FIXUP] = 1;
+            S->sc[S_RES] = snorm / bnorm;
+        }
+        return;
+    }
+    const double om = tot[1] > 0.0 ? tot[2] / tot[1] : 0.0;
+    if (lead) S->sc[S_OMEGA] = om;
+    const double alpha = S->sc[S_ALPHA];
     const int cur = S->st[I_CUR];
     const double* xc = cur ? x1 : x0;
     double* xo = cur ? x0 : x1;
@@ -244,7 +237,7 @@ __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const double* __restri
         acc[0] += ri * ri;
         acc[1] += rh[i] * ri;
     }
-    block_partials<2>(acc, rb.partials);
+    block_partials<2>(acc, part_out);
 }
 
 __global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int nparts, State* S, double tol, int k,
@@ -298,10 +291,13 @@ __global__ void __launch_bounds__(kBlock) bi_final(int64_t n, double* __restrict
 }
 
 struct BiWork {
-    double *r, *rh, *p, *v, *ph, *s, *sh, *t, *x1;
+    double *r, *rh, *p[2], *v[2], *ph, *s, *sh, *t, *x1;
     State* S;
-    RedBuf rb;
+    unsigned* counter;
+    double* part[2];
 };
+
+constexpr int64_t kPartBytes = (int64_t)kRedGrid * kMaxRed * 8;
 
 static BiWork carve_bi(int64_t n, void* work) {
     char* w = (char*)work;
@@ -309,24 +305,27 @@ static BiWork carve_bi(int64_t n, void* work) {
     double* d = (double*)w;
     W.r = d;
     W.rh = d + n;
-    W.p = d + 2 * n;
-    W.v = d + 3 * n;
-    W.ph = d + 4 * n;
-    W.s = d + 5 * n;
-    W.sh = d + 6 * n;
-    W.t = d + 7 * n;
-    W.x1 = d + 8 * n;
-    char* tail = w + ((9 * n * sizeof(double) + 255) / 256) * 256;
+    W.p[0] = d + 2 * n;
+    W.p[1] = d + 3 * n;
+    W.v[0] = d + 4 * n;
+    W.v[1] = d + 5 * n;
+    W.ph = d + 6 * n;
+    W.s = d + 7 * n;
+    W.sh = d + 8 * n;
+    W.t = d + 9 * n;
+    W.x1 = d + 10 * n;
+    char* tail = w + ((11 * n * sizeof(double) + 255) / 256) * 256;
     W.S = (State*)tail;
-    W.rb.counter = (unsigned*)(tail + 256);
-    W.rb.partials = (double*)(tail + 512);
+    W.counter = (unsigned*)(tail + 256);
+    W.part[0] = (double*)(tail + 512);
+    W.part[1] = (double*)(tail + 512 + kPartBytes);
     return W;
 }
 
 }  // namespace
 
 extern "C" int64_t ftmi_bicgstab_work_bytes(int64_t n) {
-    return ((9 * n * (int64_t)sizeof(double) + 255) / 256) * 256 + 512 + (int64_t)kMaxGrid * kMaxRed * 8;
+    return ((11 * n * (int64_t)sizeof(double) + 255) / 256) * 256 + 512 + 2 * kPartBytes;
 }
 
 namespace ftmi {
@@ -341,24 +340,21 @@ int bicgstab_run(int64_t n, Op A, const double* b, const double* inv, double* x,
         return -1;
     }
     BiWork W = carve_bi(n, work);
-    const int g = grid_for(n);
-    k_state_reset<<<1, 64, 0, st>>>(W.S, W.rb.counter);
-    bi_init<Op><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p, W.v, W.S, W.rb, tol, rel);
-    fin_bi_init<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, tol, rel);
+    const int g = grid_red(n);
+    k_state_reset<<<1, 64, 0, st>>>(W.S, W.counter);
+    bi_init<Op><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
+    fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel);
     FTMI_LAUNCH_CHECK("bicgstab init");
     if (post_state(pn, 0, W.S, st)) return -1;
     // slot `last` holds the most recently posted state; the loop waits one iteration behind
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
-        bi_p<<<g, kBlock, 0, st>>>(n, W.r, W.p, W.v, inv, W.ph, W.S);
-        bi_v<Op><<<g, kBlock, 0, st>>>(n, A, W.ph, W.rh, W.v, W.S, W.rb, (int)k);
-        fin_bi_v<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, (int)k);
-        bi_s<<<g, kBlock, 0, st>>>(n, W.r, W.v, inv, W.s, W.sh, W.S, W.rb, tol, (int)k);
-        fin_bi_s<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, tol, (int)k);
-        bi_t<Op><<<g, kBlock, 0, st>>>(n, A, W.sh, W.s, W.t, W.S, W.rb);
-        fin_bi_t<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S);
-        bi_r<<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.rb, tol, (int)k, (int)maxiter);
-        fin_bi_r<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, tol, (int)k, (int)maxiter);
+        const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+        bi_pv<Op><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S, W.part[0]);
+        bi_st<Op><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1], (int)k);
+        bi_r<<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0], tol,
+                                   (int)k);
+        fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter);
         FTMI_LAUNCH_CHECK("bicgstab iteration");
         if (post_state(pn, last ^ 1, W.S, st)) return -1;
         if (wait_state(pn, last)) return -1;  // the previous iteration's flags
@@ -381,7 +377,7 @@ int bicgstab_run(int64_t n, Op A, const double* b, const double* inv, double* x,
         conv = 0;
     }
     if (H.st[I_CUR] || H.st[I_FIXUP])
-        bi_final<<<g, kBlock, 0, st>>>(n, x, W.x1, W.ph, H.sc[S_ALPHA], H.st[I_CUR], H.st[I_FIXUP]);
+        bi_final<<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, H.sc[S_ALPHA], H.st[I_CUR], H.st[I_FIXUP]);
     FTMI_LAUNCH_CHECK("bicgstab final");
     return conv;
 }
@@ -523,9 +519,8 @@ __global__ void __launch_bounds__(kBlock) vc_pre1(int64_t n, const double* __res
 template <int TPR, bool SUM>
 __global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<double> A, const double* __restrict__ Dinv,
                                                     const double* __restrict__ b, const double* __restrict__ x,
-                                                    double* __restrict__ y, double w, const int* done, RedBuf rb,
-                                                    double* mean_out) {
-    (void)mean_out;
+                                                    double* __restrict__ y, double w, const int* done,
+                                                    double* sum_partials) {
     if (done && *done) return;
     RowGroups<TPR> G;
     double acc[1] = {0.0};
@@ -537,7 +532,7 @@ __global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<double> A, 
             if (SUM) acc[0] += yi;
         }
     }
-    if (SUM) block_partials<1>(acc, rb.partials);
+    if (SUM) block_partials<1>(acc, sum_partials);
 }
 
 template <int TPR>
@@ -582,24 +577,15 @@ __global__ void __launch_bounds__(kBlock) vc_copy(int64_t n, const double* __res
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) b[i] = a[i];
 }
 
-// *mean_out = sum(z) / n
-__global__ void __launch_bounds__(kBlock) k_mean(int64_t n, const double* __restrict__ z, const int* done, RedBuf rb,
-                                                 double* mean_out) {
-    (void)mean_out;
+// workgroup partial sums of z (the PCG consumer turns them into the mean)
+__global__ void __launch_bounds__(kBlock) k_sum(int64_t n, const double* __restrict__ z, const int* done,
+                                                double* sum_partials) {
     if (done && *done) return;
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[1] = {0.0};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc[0] += z[i];
-    block_partials<1>(acc, rb.partials);
-}
-
-// *mean_out = (sum of the partials) / n, unless the solve is done
-__global__ void __launch_bounds__(kBlock) fin_mean(const double* partials, int nparts, int64_t n, const int* done,
-                                                   double* mean_out) {
-    if (done && *done) return;
-    double tot[1];
-    if (finalize_sum<1>(partials, nparts, tot)) *mean_out = tot[0] / (double)n;
+    block_partials<1>(acc, sum_partials);
 }
 
 // x = Minv b for the dense coarse inverse (row per wave)
@@ -626,31 +612,34 @@ __global__ void __launch_bounds__(kBlock) vc_coarse(int64_t n, const double* __r
         }                                                                                                \
     } while (0)
 
+// grid of the summing smoother: its partials are consumed by every workgroup of the next kernel
+inline int sum_grid(int64_t n, int tpr) {
+    const int g = grid_rows(n, tpr);
+    return g < kRedGrid ? g : kRedGrid;
+}
+
 template <int TPR>
 static void launch_jacobi(int64_t n, CsrOp<double> A, const double* Dinv, const double* b, const double* x,
-                          double* y, double w, const int* done, RedBuf* rb, double* mean_out, hipStream_t st) {
-    const int g = grid_rows(n, TPR);
-    if (rb) {
-        vc_jacobi<TPR, true><<<g, kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, *rb, mean_out);
-        fin_mean<<<1, kBlock, 0, st>>>(rb->partials, g, n, done, mean_out);
-    } else {
-        vc_jacobi<TPR, false><<<g, kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, RedBuf{}, nullptr);
-    }
+                          double* y, double w, const int* done, double* zsum, hipStream_t st) {
+    if (zsum)
+        vc_jacobi<TPR, true><<<sum_grid(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, zsum);
+    else
+        vc_jacobi<TPR, false><<<grid_rows(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, nullptr);
 }
 
 static void jacobi(int tpr, int64_t n, CsrOp<double> A, const double* Dinv, const double* b, const double* x,
-                   double* y, double w, const int* done, RedBuf* rb, double* mean_out, hipStream_t st) {
+                   double* y, double w, const int* done, double* zsum, hipStream_t st) {
     switch (tpr) {
-        case 1: launch_jacobi<1>(n, A, Dinv, b, x, y, w, done, rb, mean_out, st); break;
-        case 4: launch_jacobi<4>(n, A, Dinv, b, x, y, w, done, rb, mean_out, st); break;
-        case 16: launch_jacobi<16>(n, A, Dinv, b, x, y, w, done, rb, mean_out, st); break;
-        default: launch_jacobi<64>(n, A, Dinv, b, x, y, w, done, rb, mean_out, st); break;
+        case 1: launch_jacobi<1>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        case 4: launch_jacobi<4>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        case 16: launch_jacobi<16>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        default: launch_jacobi<64>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
     }
 }
 
-// Enqueue one V-cycle on level k: x_k = M_k^{-1} b_k. When `rb` is given, the sum of the
-// level-0 output is reduced into *mean_out (divided by n).
-static void vcycle(const Amg& M, int k, hipStream_t st, const int* done, RedBuf* rb, double* mean_out) {
+// Enqueue one V-cycle on level k: x_k = M_k^{-1} b_k. When `zsum` is given, the last
+// smoothing sweep also writes the workgroup partial sums of the output (sum_grid of them).
+static void vcycle(const Amg& M, int k, hipStream_t st, const int* done, double* zsum) {
     if (k == M.nlev) {
         const int rows_per_block = kBlock / kWave;
         vc_coarse<<<(int)((M.nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M.nc, M.cinv, M.cb, M.cx,
@@ -668,7 +657,7 @@ static void vcycle(const Amg& M, int k, hipStream_t st, const int* done, RedBuf*
     if (M.sweeps >= 2) {
         TPR_LAUNCH(L.tpr_a, vc_pre2, L.n, st, L.n, L.A, L.Dinv, L.b, xa, w, done);
         for (int s = 2; s < M.sweeps; ++s) {
-            jacobi(L.tpr_a, L.n, L.A, L.Dinv, L.b, xa, xb, w, done, nullptr, nullptr, st);
+            jacobi(L.tpr_a, L.n, L.A, L.Dinv, L.b, xa, xb, w, done, nullptr, st);
             double* t = xa;
             xa = xb;
             xb = t;
@@ -678,14 +667,14 @@ static void vcycle(const Amg& M, int k, hipStream_t st, const int* done, RedBuf*
     }
     TPR_LAUNCH(L.tpr_a, vc_resid, L.n, st, L.n, L.A, L.b, xa, L.r, done);
     TPR_LAUNCH(L.tpr_r, vc_spmv, next_n, st, next_n, L.R, L.r, next_b, done);
-    vcycle(M, k + 1, st, done, nullptr, nullptr);
+    vcycle(M, k + 1, st, done, nullptr);
     TPR_LAUNCH(L.tpr_p, vc_prolong, L.n, st, L.n, L.P, next_x, xa, done);
     // post-smoothing: ping-pong between the level's two buffers, never in place
     double* cur = xa;
     double* other = (xa == L.x) ? L.y : L.x;
     for (int s = 0; s < M.sweeps; ++s) {
         const bool last = (s == M.sweeps - 1);
-        jacobi(L.tpr_a, L.n, L.A, L.Dinv, L.b, cur, other, w, done, last ? rb : nullptr, mean_out, st);
+        jacobi(L.tpr_a, L.n, L.A, L.Dinv, L.b, cur, other, w, done, last ? zsum : nullptr, st);
         double* t = cur;
         cur = other;
         other = t;
@@ -768,7 +757,7 @@ int ftmi_amg_vcycle(void* h, const double* b, double* x, void* stream) {
     double *sb = L0.b, *sx = L0.x;
     L0.b = (double*)b;
     L0.x = x;
-    vcycle(*M, 0, st, nullptr, nullptr, nullptr);
+    vcycle(*M, 0, st, nullptr, nullptr);
     L0.b = sb;
     L0.x = sx;
     FTMI_LAUNCH_CHECK("ftmi_amg_vcycle");
@@ -778,10 +767,13 @@ int ftmi_amg_vcycle(void* h, const double* b, double* x, void* stream) {
 }  // extern "C"
 
 // ====================================================================== PCG
+// One iteration: q = A p with the p update fused in (p = z + beta p_prev, recomputed at the
+// neighbours), the x/r update, the state finalize the host polls, the V-cycle, and the
+// projection of z with r.z. p alternates between two buffers.
 namespace {
 
 template <bool PROJ>
-__global__ void __launch_bounds__(kBlock) pcg_init_a(int64_t n, const double* __restrict__ b, State* S, RedBuf rb) {
+__global__ void __launch_bounds__(kBlock) pcg_init_a(int64_t n, const double* __restrict__ b, double* part_out) {
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[2] = {0.0, 0.0};
@@ -789,23 +781,22 @@ __global__ void __launch_bounds__(kBlock) pcg_init_a(int64_t n, const double* __
         acc[0] += b[i];
         acc[1] += b[i] * b[i];
     }
-    block_partials<2>(acc, rb.partials);
+    block_partials<2>(acc, part_out);
 }
 
-template <bool PROJ>
-__global__ void __launch_bounds__(kBlock) fin_pcg_init_a(const double* partials, int nparts, int64_t n, State* S) {
-    double tot[2];
-    if (finalize_sum<2>(partials, nparts, tot)) {
-        S->sc[S_BMEAN] = PROJ ? tot[0] / (double)n : 0.0;
-        const double bn = sqrt(tot[1]);
-        S->sc[S_BNORM] = bn != 0.0 ? bn : 1.0;
-    }
-}
-
+// bnorm, mean(b); r = b - mean (projection) or b; x = 0; partials r.r
 template <bool PROJ>
 __global__ void __launch_bounds__(kBlock) pcg_init_b(int64_t n, const double* __restrict__ b, double* __restrict__ r,
-                                                     double* __restrict__ x, State* S, RedBuf rb, double tol) {
-    const double mean = S->sc[S_BMEAN];
+                                                     double* __restrict__ x, State* S, const double* part_in,
+                                                     int np_in, double* part_out) {
+    double tot[2];
+    consume_sum<2>(part_in, np_in, tot);
+    const double mean = PROJ ? tot[0] / (double)n : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double bn = sqrt(tot[1]);
+        S->sc[S_BNORM] = bn != 0.0 ? bn : 1.0;
+        S->sc[S_BMEAN] = mean;
+    }
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[1] = {0.0};
@@ -815,10 +806,10 @@ __global__ void __launch_bounds__(kBlock) pcg_init_b(int64_t n, const double* __
         x[i] = 0.0;
         acc[0] += ri * ri;
     }
-    block_partials<1>(acc, rb.partials);
+    block_partials<1>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_pcg_init_b(const double* partials, int nparts, State* S, double tol) {
+__global__ void __launch_bounds__(kBlock) fin_pcg_init(const double* partials, int nparts, State* S, double tol) {
     double tot[1];
     if (finalize_sum<1>(partials, nparts, tot)) {
         const double rn = sqrt(tot[0]);
@@ -832,12 +823,17 @@ __global__ void __launch_bounds__(kBlock) fin_pcg_init_b(const double* partials,
     }
 }
 
-// z -= mean(z) (projection); rz_new = r.z; first: p = z, rz = rz_new; else beta = rz_new / rz.
+// z -= mean(z) (mean from the V-cycle's partial sums); partials r.z
 template <bool PROJ>
 __global__ void __launch_bounds__(kBlock) pcg_z(int64_t n, const double* __restrict__ r, double* __restrict__ z,
-                                                double* __restrict__ p, State* S, RedBuf rb, int first, int k) {
+                                                const State* S, const double* zsum_in, int np_zsum, double* part_out) {
     if (S->st[I_DONE]) return;
-    const double mean = S->sc[S_ZMEAN];
+    double mean = 0.0;
+    if (PROJ) {
+        double tot[1];
+        consume_sum<1>(zsum_in, np_zsum, tot);
+        mean = tot[0] / (double)n;
+    }
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[1] = {0.0};
@@ -847,73 +843,68 @@ __global__ void __launch_bounds__(kBlock) pcg_z(int64_t n, const double* __restr
             zi = zi - mean;
             z[i] = zi;
         }
-        if (first) p[i] = zi;
         acc[0] += r[i] * zi;
     }
-    block_partials<1>(acc, rb.partials);
+    block_partials<1>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_pcg_z(const double* partials, int nparts, State* S, int first, int k) {
+// rz_new from the partials; first: p = z, rz = rz_new; else breakdown if rz == 0, beta =
+// rz_new / rz, p = z + beta p_prev; then q = A p and partials p.q
+template <class Op>
+__global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const double* __restrict__ z,
+                                                 const double* __restrict__ p_prev, double* __restrict__ p,
+                                                 double* __restrict__ q, State* S, const double* part_in, int np_in,
+                                                 double* part_out, int first, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
-    if (finalize_sum<1>(partials, nparts, tot)) {
-        double* sc = S->sc;
-        if (first) {
-            sc[S_RZ] = tot[0];
-        } else if (sc[S_RZ] == 0.0) {
-            S->st[I_DONE] = 1;
-            S->st[I_CONV] = 0;
-            S->st[I_ITERS] = k;
-        } else {
-            sc[S_BETA] = tot[0] / sc[S_RZ];
-            sc[S_RZ] = tot[0];
+    consume_sum<1>(part_in, np_in, tot);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const int cur = S_RZ + (k & 1), prev = S_RZ + ((k - 1) & 1);
+    double beta = 0.0;
+    if (!first) {
+        const double rz = S->sc[prev];
+        if (rz == 0.0) {  // breakdown found at the end of the previous iteration
+            if (lead) {
+                S->st[I_DONE] = 1;
+                S->st[I_CONV] = 0;
+                S->st[I_ITERS] = k - 1;
+            }
+            return;
         }
+        beta = tot[0] / rz;
     }
-}
-
-__global__ void __launch_bounds__(kBlock) pcg_p(int64_t n, const double* __restrict__ z, double* __restrict__ p,
-                                                const State* S) {
-    if (S->st[I_DONE]) return;
-    const double beta = S->sc[S_BETA];
-    int64_t lo, hi;
-    block_range(n, lo, hi);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) p[i] = z[i] + beta * p[i];
-}
-
-template <class Op>
-__global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const double* __restrict__ p,
-                                                 double* __restrict__ q, State* S, RedBuf rb, int k) {
-    if (S->st[I_DONE]) return;
+    if (lead) S->sc[cur] = tot[0];
+    auto pf = [&](int64_t j) { return first ? z[j] : z[j] + beta * p_prev[j]; };
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[1] = {0.0};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double qi = A.row(i, p);
+        const double pi = pf(i);
+        p[i] = pi;
+        const double qi = A.row_f(i, pf);
         q[i] = qi;
-        acc[0] += p[i] * qi;
+        acc[0] += pi * qi;
     }
-    block_partials<1>(acc, rb.partials);
+    block_partials<1>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_pcg_ap(const double* partials, int nparts, State* S, int k) {
+// alpha = rz / (p.q) (breakdown if p.q <= 0); x += alpha p; r -= alpha q; partials r.r
+__global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const double* __restrict__ p, const double* __restrict__ q,
+                                                 double* __restrict__ x, double* __restrict__ r, State* S,
+                                                 const double* part_in, int np_in, double* part_out, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
-    if (finalize_sum<1>(partials, nparts, tot)) {
-        if (tot[0] <= 0.0) {
+    consume_sum<1>(part_in, np_in, tot);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (tot[0] <= 0.0) {
+        if (lead) {
             S->st[I_DONE] = 1;
             S->st[I_CONV] = 0;
             S->st[I_ITERS] = k;
-        } else {
-            S->sc[S_ALPHA] = S->sc[S_RZ] / tot[0];
         }
+        return;
     }
-}
-
-__global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const double* __restrict__ p, const double* __restrict__ q,
-                                                 double* __restrict__ x, double* __restrict__ r, State* S, RedBuf rb,
-                                                 double tol, int k) {
-    if (S->st[I_DONE]) return;
-    const double a = S->sc[S_ALPHA];
+    const double a = S->sc[S_RZ + (k & 1)] / tot[0];
     int64_t lo, hi;
     block_range(n, lo, hi);
     double acc[1] = {0.0};
@@ -923,7 +914,7 @@ __global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const double* __rest
         r[i] = ri;
         acc[0] += ri * ri;
     }
-    block_partials<1>(acc, rb.partials);
+    block_partials<1>(acc, part_out);
 }
 
 __global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int nparts, State* S, double tol, int k) {
@@ -942,9 +933,10 @@ __global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int
 }
 
 struct PcgWork {
-    double *r, *z, *p, *q;
+    double *r, *z, *p[2], *q;
     State* S;
-    RedBuf rb;
+    unsigned* counter;
+    double* part[3];
 };
 
 static PcgWork carve_pcg(int64_t n, void* work) {
@@ -953,12 +945,13 @@ static PcgWork carve_pcg(int64_t n, void* work) {
     double* d = (double*)w;
     W.r = d;
     W.z = d + n;
-    W.p = d + 2 * n;
-    W.q = d + 3 * n;
-    char* tail = w + ((4 * n * sizeof(double) + 255) / 256) * 256;
+    W.p[0] = d + 2 * n;
+    W.p[1] = d + 3 * n;
+    W.q = d + 4 * n;
+    char* tail = w + ((5 * n * sizeof(double) + 255) / 256) * 256;
     W.S = (State*)tail;
-    W.rb.counter = (unsigned*)(tail + 256);
-    W.rb.partials = (double*)(tail + 512);
+    W.counter = (unsigned*)(tail + 256);
+    for (int j = 0; j < 3; ++j) W.part[j] = (double*)(tail + 512 + j * kPartBytes);
     return W;
 }
 
@@ -972,62 +965,57 @@ int pcg_run(int64_t n, Op A, const double* b, double* x, double tol, int64_t max
     }
     Amg* M = (Amg*)amg;
     PcgWork W = carve_pcg(n, work);
-    const int g = grid_for(n);
+    const int g = grid_red(n);
     const int* done = &W.S->st[I_DONE];
-    k_state_reset<<<1, 64, 0, st>>>(W.S, W.rb.counter);
+    double *P0 = W.part[0], *P1 = W.part[1], *P2 = W.part[2];
+    k_state_reset<<<1, 64, 0, st>>>(W.S, W.counter);
     if (project) {
-        pcg_init_a<true><<<g, kBlock, 0, st>>>(n, b, W.S, W.rb);
-        fin_pcg_init_a<true><<<1, kBlock, 0, st>>>(W.rb.partials, g, n, W.S);
-        pcg_init_b<true><<<g, kBlock, 0, st>>>(n, b, W.r, x, W.S, W.rb, tol);
+        pcg_init_a<true><<<g, kBlock, 0, st>>>(n, b, P0);
+        pcg_init_b<true><<<g, kBlock, 0, st>>>(n, b, W.r, x, W.S, P0, g, P1);
     } else {
-        pcg_init_a<false><<<g, kBlock, 0, st>>>(n, b, W.S, W.rb);
-        fin_pcg_init_a<false><<<1, kBlock, 0, st>>>(W.rb.partials, g, n, W.S);
-        pcg_init_b<false><<<g, kBlock, 0, st>>>(n, b, W.r, x, W.S, W.rb, tol);
+        pcg_init_a<false><<<g, kBlock, 0, st>>>(n, b, P0);
+        pcg_init_b<false><<<g, kBlock, 0, st>>>(n, b, W.r, x, W.S, P0, g, P1);
     }
-    fin_pcg_init_b<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, tol);
-    auto precond = [&]() {
-        bool fused_mean = false;
+    fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol);
+    // z = M r, with the partial sums of z in P2 when projecting; returns their count
+    auto precond = [&]() -> int {
         if (M && M->nlev > 0) {
             Level& L0 = M->lv[0];
             double *sb = L0.b, *sx = L0.x;
             L0.b = W.r;
             L0.x = W.z;
-            vcycle(*M, 0, st, done, project ? &W.rb : nullptr, &W.S->sc[S_ZMEAN]);
+            vcycle(*M, 0, st, done, project ? P2 : nullptr);
             L0.b = sb;
             L0.x = sx;
-            fused_mean = project != 0;
-        } else if (M) {
+            return sum_grid(n, L0.tpr_a);
+        }
+        if (M) {
             const int rpb = kBlock / kWave;
             vc_coarse<<<(int)((M->nc + rpb - 1) / rpb), kBlock, 0, st>>>(M->nc, M->cinv, W.r, W.z, done);
         } else {
-            vc_copy<<<g, kBlock, 0, st>>>(n, W.r, W.z, done);
+            vc_copy<<<grid_for(n), kBlock, 0, st>>>(n, W.r, W.z, done);
         }
-        if (project && !fused_mean) {
-            k_mean<<<g, kBlock, 0, st>>>(n, W.z, done, W.rb, &W.S->sc[S_ZMEAN]);
-            fin_mean<<<1, kBlock, 0, st>>>(W.rb.partials, g, n, done, &W.S->sc[S_ZMEAN]);
-        }
+        if (project) k_sum<<<g, kBlock, 0, st>>>(n, W.z, done, P2);
+        return g;
     };
-    precond();
+    int nz = precond();
     if (project)
-        pcg_z<true><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.p, W.S, W.rb, 1, 0);
+        pcg_z<true><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
     else
-        pcg_z<false><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.p, W.S, W.rb, 1, 0);
-    fin_pcg_z<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, 1, 0);
+        pcg_z<false><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
     FTMI_LAUNCH_CHECK("pcg init");
     if (post_state(pn, 0, W.S, st)) return -1;
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
-        pcg_ap<Op><<<g, kBlock, 0, st>>>(n, A, W.p, W.q, W.S, W.rb, (int)k);
-        fin_pcg_ap<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, (int)k);
-        pcg_xr<<<g, kBlock, 0, st>>>(n, W.p, W.q, x, W.r, W.S, W.rb, tol, (int)k);
-        fin_pcg_xr<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, tol, (int)k);
-        precond();
+        const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+        pcg_ap<Op><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
+        pcg_xr<<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
+        fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k);
+        nz = precond();
         if (project)
-            pcg_z<true><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.p, W.S, W.rb, 0, (int)k);
+            pcg_z<true><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         else
-            pcg_z<false><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.p, W.S, W.rb, 0, (int)k);
-        fin_pcg_z<<<1, kBlock, 0, st>>>(W.rb.partials, g, W.S, 0, (int)k);
-        pcg_p<<<g, kBlock, 0, st>>>(n, W.z, W.p, W.S);
+            pcg_z<false><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         FTMI_LAUNCH_CHECK("pcg iteration");
         if (post_state(pn, last ^ 1, W.S, st)) return -1;
         if (wait_state(pn, last)) return -1;
@@ -1053,7 +1041,7 @@ int pcg_run(int64_t n, Op A, const double* b, double* x, double tol, int64_t max
 extern "C" {
 
 int64_t ftmi_pcg_work_bytes(int64_t n) {
-    return ((4 * n * (int64_t)sizeof(double) + 255) / 256) * 256 + 512 + (int64_t)kMaxGrid * kMaxRed * 8;
+    return ((5 * n * (int64_t)sizeof(double) + 255) / 256) * 256 + 512 + 3 * kPartBytes;
 }
 
 int ftmi_pcg_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,

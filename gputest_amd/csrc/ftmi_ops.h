@@ -17,7 +17,14 @@ struct CsrOp {
         for (int32_t j = ptr[i]; j < e; ++j) s += val[j] * x[col[j]];
         return s;
     }
-    __device__ inline T diag_free_row(int64_t i, const T* __restrict__ x) const { return row(i, x); }
+    // sum_j a_ij f(j), with f giving x_j on the fly
+    template <class F>
+    __device__ inline T row_f(int64_t i, F f) const {
+        T s = T(0);
+        const int32_t e = ptr[i + 1];
+        for (int32_t j = ptr[i]; j < e; ++j) s += val[j] * f(col[j]);
+        return s;
+    }
 };
 
 // Cell-cell operator of a face discretisation: a diagonal and one coefficient per
@@ -33,6 +40,13 @@ struct LduOp {
         T s = diag[i] * x[i];
         const int32_t e = ptr[i + 1];
         for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * x[nbr[k]];
+        return s;
+    }
+    template <class F>
+    __device__ inline T row_f(int64_t i, F f) const {
+        T s = diag[i] * f(i);
+        const int32_t e = ptr[i + 1];
+        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * f(nbr[k]);
         return s;
     }
 };

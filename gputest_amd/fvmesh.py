@@ -192,6 +192,7 @@ class FVMesh:
         self.bptr, self.bface = i32(bptr), i32(bface)
         self.indptr, self.indices, self.slot_diag = i32(indptr), i32(indices), i32(slot_diag)
         self.src_ptr, self.src = i32(src_ptr), i32(src)
+        self.he_slot = i32(he_slot)
         # the reference's incidence adjacency for the drop-in kernels (iidx sorted = cface >> 1)
         self.iidx = i32(cface >> 1)
         self.isgn = f64(np.where(cface & 1, -1.0, 1.0))

@@ -255,8 +255,7 @@ class SimpleSolver:
         # 6. pressure correction
         if self._amg is None or self._amg_age >= self.amg_rebuild:
             t0 = time.perf_counter()
-            Ah = fv.ldu_to_csr_host(self.diag_p.cpu().numpy(), self.offp.cpu().numpy())
-            self._amg = SmoothedAggregationAMG(Ah, self.device, singular=self.singular_p)
+            self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
         self._amg_age += 1

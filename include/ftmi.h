@@ -118,6 +118,40 @@ int ftmi_amg_vcycle(void* amg, const double* b, double* x, void* stream);
 /* host helper (host pointers): three-pass greedy aggregation of amg.py:26-57 over a strength
  * graph in CSR form; returns the number of aggregates */
 int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* idx, int64_t* agg);
+/* host helper: the same aggregation over a CSR pattern with a strength mask per entry
+ * (int32 in/out); size[a] = members of aggregate a (size holds n entries) */
+int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
+                                     int32_t* agg, int32_t* size);
+
+/* new: SA-AMG set-up on the device (amg.py:60-117, everything but the aggregation) */
+/* CSR data of the pressure matrix from its face form (he_slot: CSR slot of each half-edge) */
+int ftmi_amgs_ldu_to_csr(int64_t n, const int32_t* cptr, const int32_t* he_slot, const int32_t* slot_diag,
+                         const int32_t* indptr, const double* diag, const double* off, double* data, void* stream);
+/* diagonal and its inverse */
+int ftmi_amgs_diag(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, double* diag, double* dinv,
+                   void* stream);
+/* strength mask |a_ij| >= theta sqrt(|a_ii a_jj|), i != j (amg.py:60-66) */
+int ftmi_amgs_strength(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* diag,
+                       double theta, uint8_t* mask, void* stream);
+/* tentative prolongator as CSR: t_ptr = 0..n, t_val = 1/sqrt(|aggregate|) */
+int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, int32_t* t_ptr, double* t_val,
+                        void* stream);
+/* rho(D^-1 A) by power iteration from x (amg.py:69-77); out[0] = the estimate (device) */
+int ftmi_amgs_power(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* dinv,
+                    double* x, double* y, int iters, double* partials, double* out, void* stream);
+/* deterministic sparse product C = diag(xscale) X Y, count (fill = 0) or fill pass; with agg:
+ * C = T - (omega_scale / *omega_num) diag(xscale) X Y (the smoothed prolongator) */
+int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_t* x_col, const double* x_val,
+                     const int32_t* y_ptr, const int32_t* y_col, const double* y_val, const double* xscale,
+                     const int32_t* agg, const double* tval, const double* omega_num, double omega_scale,
+                     int32_t* rowcnt, const int32_t* c_ptr, int32_t* c_col, double* c_val, int* overflow,
+                     void* stream);
+/* exclusive scan: out[0..n] (out[n] = total); bsum: 2048 int64 of work */
+int ftmi_scan_i32(int64_t n, const int32_t* in, int32_t* out, int64_t* bsum, void* stream);
+/* transpose: column counts, then placement and per-row sort */
+int ftmi_amgs_count_cols(int64_t nnz, const int32_t* col, int32_t* cnt, void* stream);
+int ftmi_amgs_transpose_fill(int64_t nrows, int64_t ncols, const int32_t* ptr, const int32_t* col, const double* val,
+                             const int32_t* r_ptr, int32_t* fill, int32_t* r_col, double* r_val, void* stream);
 
 /* ---------------------------------------------------------------- fused SIMPLEC stages */
 

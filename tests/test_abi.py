@@ -26,6 +26,39 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def declared_prototypes():
+    """{name: [parameter types]} from include/ftmi.h."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(ftmi_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        params = [p.strip() for p in m.group(2).split(",") if p.strip() and p.strip() != "void"]
+        out[m.group(1)] = params
+    return out
+
+
+def _ctype_of(param):
+    import ctypes as ct
+
+    if "*" in param:
+        return ct.c_void_p
+    base = param.rsplit(" ", 1)[0].replace("const", "").strip()
+    return {"int": ct.c_int, "int64_t": ct.c_int64, "double": ct.c_double}[base]
+
+
+def test_binding_matches_header_prototypes():
+    """Every ctypes signature has the header's parameter count and kinds (pointer / int /
+    int64 / double), so a binding mistake fails here instead of on the GPU."""
+    from gputest_amd import _native as nat
+
+    protos = declared_prototypes()
+    assert set(protos) == set(nat._SIGS)
+    for name, params in protos.items():
+        args, _ = nat._SIGS[name]
+        assert len(args) == len(params), (name, len(args), len(params))
+        for a, p in zip(args, params):
+            assert a == _ctype_of(p), (name, p, a)
+
+
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libftmi.so not built")
 def test_abi_version_and_binding():
     from gputest_amd import _native as nat

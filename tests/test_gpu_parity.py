@@ -111,6 +111,22 @@ def test_amg_hierarchy_vcycle_and_pcg():
     assert M.n_levels == R.n_levels == int(g["n_levels"])
     for k in range(len(M.aggregates)):
         assert np.array_equal(M.aggregates[k], g[f"L{k}_agg"])
+        # the smoothed prolongator built on the device equals the reference's
+        P = M.prolongator(k)
+        Pr = sp.csr_matrix((g[f"L{k}_P_data"], g[f"L{k}_P_indices"], g[f"L{k}_P_indptr"]), shape=P.shape)
+        Pr.sort_indices()  # the reference's P is not in canonical (sorted) order
+        assert np.array_equal(P.indptr, Pr.indptr) and np.array_equal(P.indices, Pr.indices)
+        assert rel(P.data, Pr.data) < 1e-13
+        if f"L{k}_rho" in g.files:
+            assert abs(float(h(M.rhos[k])[0]) - float(g[f"L{k}_rho"])) <= 1e-12 * float(g[f"L{k}_rho"])
+        if f"L{k + 1}_A_data" in g.files and k + 1 < len(M.levels):
+            Ac = M.levels[k + 1][0].to_scipy()
+            Ar = sp.csr_matrix((g[f"L{k + 1}_A_data"], g[f"L{k + 1}_A_indices"], g[f"L{k + 1}_A_indptr"]),
+                               shape=Ac.shape)
+            Ar.sort_indices()
+            assert np.array_equal(Ac.indptr, Ar.indptr) and np.array_equal(Ac.indices, Ar.indices)
+            assert rel(Ac.data, Ar.data) < 1e-12
+    assert rel(M.coarse_inv_h, g["coarse_inv"]) < 1e-8
     x = h(M(t(g["vcycle_b"])))
     assert rel(x, g["vcycle_x"]) < 1e-10
     assert rel(x, R(g["vcycle_b"])) < 1e-12
@@ -196,27 +212,46 @@ def test_cavity_300_iterations_within_1e8_of_reference():
     assert np.array_equal(pits, g["p_iters"])
 
 
-def _oracle_self_sensitivity(mesh, case, g, eps=1e-15):
-    """How far the oracle itself moves from the reference trajectory when its initial face
-    fluxes carry noise of size eps: the scale of round-off amplification in this case."""
+def _errors(U, p, F, T, g, singular):
+    """Errors against the reference trajectory: relative L2 per velocity component (absolute,
+    scaled by max |U|, for a component that is round-off noise), pressure as max |dp| over
+    max(max |p|, max |U|^2), flux and temperature as relative L2."""
+    gU, gp = g["U"], g["p"]
+    if singular:
+        p, gp = p - p.mean(), gp - gp.mean()
+    umax = float(np.abs(gU).max())
+    out = {}
+    for k in range(U.shape[1]):
+        nrm = np.linalg.norm(gU[:, k])
+        out[f"U{k}"] = rel_l2(U[:, k], gU[:, k]) if nrm > 1e-6 * np.linalg.norm(gU) else \
+            float(np.abs(U[:, k] - gU[:, k]).max()) / umax
+    out["p"] = float(np.abs(p - gp).max()) / max(float(np.abs(gp).max()), umax ** 2)
+    out["F"] = rel_l2(F, g["F"])
+    if T is not None:
+        out["T"] = rel_l2(T, g["T"])
+    return out
+
+
+def _oracle_ulp_sensitivity(mesh, case, g):
+    """The oracle's own drift from the reference trajectory when every face flux moves by one
+    ulp after the first iteration: the round-off amplification of the algorithm on this case."""
     s = O.Simplec(mesh, case)
-    rng = np.random.default_rng(11)
-    s.F = s.F + eps * rng.standard_normal(s.F.size)
+    s.step()
+    s.F = np.nextafter(s.F, np.inf)
     while not s.finished:
         s.step()
-    out = {f"U{k}": rel_l2(s.U[:, k], g["U"][:, k]) for k in range(s.dim)}
-    if s.energy:
-        out["T"] = rel_l2(s.T, g["T"])
-    return out
+    return _errors(s.U, s.p, s.F, s.T if s.energy else None, g, s.singular_p)
 
 
 @pytest.mark.parametrize("which", ["duct", "channel"])
 def test_short_runs_match_reference(which):
-    """20 iterations of reduced-size workloads at the default pressure tolerance (0.1), against
-    the reference's trajectory. Velocity components, pressure and fluxes must agree to 1e-8
-    unless the reference algorithm itself amplifies round-off beyond that on this case (the
-    noise-driven pressure coupling of the periodic channel; the energy system of the duct):
-    those are bounded by 10x the oracle's own drift under a 1e-15 perturbation."""
+    """20 iterations of reduced-size workloads at the default pressure tolerance (0.1) against
+    the reference's trajectory: 1e-8, or 10x the oracle's own one-ulp drift where the algorithm
+    amplifies round-off beyond that (the noise-driven pressure coupling of the periodic
+    channel). Temperature is not iterate-level stable: its 60-step Jacobi-BiCGSTAB does not
+    converge and amplifies round-off by orders of magnitude (the reference measured 1e-15 ->
+    1e-2 for its own native port); it is pinned by the captured iteration (1e-7, same count)
+    and by test_energy_converges_to_the_oracle_solution (1e-6), and bounded here by 1e-3."""
     if which == "duct":
         from gputest_amd.mesh import duct_mesh
 
@@ -231,25 +266,11 @@ def test_short_runs_match_reference(which):
     assert s.relax_p == float(g["relax_p"])
     while not s.finished:
         s.step()
-    sens = _oracle_self_sensitivity(mesh, case_o, g)
-    U, p = h(s.U).T, h(s.p)
-    gU, gp = g["U"], g["p"]
-    if s.singular_p:
-        p, gp = p - p.mean(), gp - gp.mean()
-    umax = float(np.abs(gU).max())
-    errs = {}
-    for k in range(U.shape[1]):
-        nrm = np.linalg.norm(gU[:, k])
-        # a velocity component that is round-off noise (v, w of the channel) is compared in absolute terms
-        errs[f"U{k}"] = rel_l2(U[:, k], gU[:, k]) if nrm > 1e-6 * np.linalg.norm(gU) else \
-            float(np.abs(U[:, k] - gU[:, k]).max()) / umax
-    errs["p"] = float(np.abs(p - gp).max()) / max(float(np.abs(gp).max()), umax ** 2)
-    errs["F"] = rel_l2(h(s.F), g["F"])
-    if s.energy:
-        errs["T"] = rel_l2(h(s.T), g["T"])
-    print(which, "errors", errs, "oracle self-sensitivity", sens)
+    errs = _errors(h(s.U).T, h(s.p), h(s.F), h(s.T) if s.energy else None, g, s.singular_p)
+    sens = _oracle_ulp_sensitivity(mesh, case_o, g)
+    print(which, "errors", errs, "oracle one-ulp drift", sens)
     for k, e in errs.items():
-        bound = max(1e-8, 10.0 * sens.get(k, 0.0))
+        bound = 1e-3 if k == "T" else max(1e-8, 10.0 * sens[k])
         assert e <= bound, (k, e, bound)
 
 
@@ -277,3 +298,37 @@ def test_energy_converges_to_the_oracle_solution():
         e = rel(f[k], r)
         print(k, e, ref.iteration, s.iteration)
         assert e < 1e-6, (k, e)
+
+
+def test_krylov_at_sizes_that_fill_the_reduction_grid():
+    """360,000 unknowns: reductions run on the capped workgroup grid and every workgroup of a
+    consuming kernel re-adds the partials (the configuration of the million-cell workload)."""
+    n1 = 600
+    T1 = sp.diags([-np.ones(n1 - 1), 2 * np.ones(n1), -np.ones(n1 - 1)], [-1, 0, 1])
+    A = (sp.kron(T1, sp.identity(n1)) + sp.kron(sp.identity(n1), T1)).tocsr()
+    from gputest_amd.fvmesh import DeviceCSR
+
+    def dev(M):
+        M = sp.csr_matrix(M)
+        M.sort_indices()
+        return DeviceCSR(t(M.data), torch.as_tensor(M.indices.astype(np.int32), device=DEV),
+                         torch.as_tensor(M.indptr.astype(np.int32), device=DEV), M.shape)
+
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(A.shape[0])
+    R = O.SAAMG(A)
+    xo, io = O.pcg(lambda v: A @ v, b, M=R, tol=1e-8, maxiter=200)
+    M = SmoothedAggregationAMG(A, DEV)
+    xd, idd = pcg_csr(dev(A), t(b), 1e-8, 200, amg=M)
+    print("pcg iterations", io.iterations, idd.iterations)
+    assert idd.converged and idd.iterations == io.iterations
+    assert rel(h(xd), xo) < 1e-6
+    N = (A + sp.diags(rng.uniform(0, 0.5, A.shape[0])) + 0.3 * sp.eye(A.shape[0], k=1)).tocsr()
+    inv = 1.0 / N.diagonal()
+    x0 = rng.standard_normal(A.shape[0])
+    xr, ir = O.bicgstab(lambda v: N @ v, b, M=lambda v: inv * v, x0=x0, tol=1e-6, maxiter=100,
+                        relative_to_initial=True)
+    xn, inn = bicgstab_jacobi(dev(N), t(b), t(x0), t(inv), 1e-6, 100)
+    print("bicgstab iterations", ir.iterations, inn.iterations)
+    assert (inn.iterations, inn.converged) == (ir.iterations, ir.converged)
+    assert rel(h(xn), xr) < 1e-8
