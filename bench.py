@@ -6,10 +6,10 @@ Re = 500, energy on (inlet T = 300, wall heat flux 5000, Pr = 0.7), FP64 SIMPLEC
 SA-AMG-PCG (pressure tolerance 0.1) and Jacobi-BiCGSTAB. A step is one outer iteration; the
 AMG hierarchy is rebuilt every 10 iterations inside the timed region, as the solver does.
 
-Prints one JSON line (see ``--help``). ``roofline`` is measured live for the fused momentum
-assembly kernel (its launches bracketed by HIP events on the solver's stream), and
-``cpu_baseline`` times the CPU FP64 restatement of the same algorithm on a bounded sample
-of the same workload (rank 0, N = 1 only).
+Prints one JSON line (see ``--help``). ``roofline`` is measured live for the dominant kernel,
+the BiCGSTAB p-update fused with the face-operator SpMV (its launches in three further outer
+iterations bracketed by HIP events on the solver's stream); ``cpu_baseline`` times the CPU
+FP64 restatement of the same algorithm on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -47,52 +47,55 @@ def _workload(nx, ny):
     return duct_mesh(nx, ny), case, parabolic
 
 
-def _assemble_bytes(s) -> int:
-    """Compulsory HBM bytes of one momentum-assembly launch (each array once)."""
-    fv, dim, nc, nf = s.fv, s.dim, s.nc, s.fv.n_if
-    cell_in = (nc + 1) * 4 * 2 + nc * 8 * (dim + dim * dim + dim + 1)  # cptr, bptr, U, gU, gp, vol
-    cell_out = nc * 8 * (3 + 2 * dim + 1)  # diag, diag_r, inv_r, b, x0, dP
-    half_edge = 2 * nf * (4 + 4 + 8)  # cface, cnbr in; off out
-    face_in = nf * 8 * (3 + 3 * dim)  # F, alpha, w; Tvec, dP_f, dN_f
-    nb = fv.n_bf
-    bnd = nb * (4 + 4 + 8 * (3 + dim))  # bface, kind, Fb, alpha_b, ub
-    return int(cell_in + cell_out + half_edge + face_in + bnd)
+def bi_pv_bytes(n: int, nf: int) -> int:
+    """Compulsory HBM bytes of one launch of the BiCGSTAB p-update + SpMV kernel on the face
+    (LDU) operator: reads r, p_prev, v_prev, inv, rhat, diag (6 n doubles), the adjacency
+    pointers (n + 1 int32), neighbours (2 nf int32) and coefficients (2 nf doubles); writes
+    p, phat, v (3 n doubles). Neighbour values gathered again count once."""
+    return int(n * 8 * 6 + (n + 1) * 4 + 2 * nf * (4 + 8) + n * 8 * 3)
 
 
-def measure_roofline(s, reps=30):
-    """Average duration of the fused momentum-assembly kernel, timed with HIP events around
-    ``reps`` back-to-back launches on the solver's (current) stream."""
+def measure_roofline(s, steps=3):
+    """Average duration of the dominant kernel (bi_pv, BiCGSTAB p-update + LDU SpMV) over the
+    launches of ``steps`` further outer iterations, from HIP events the library records around
+    each launch on the solver's stream."""
     import torch
 
     from gputest_amd import _native as nat
 
     L = nat.lib()
-    fv, dim, c = s.fv, s.dim, s.case
-    a = nat.FtmiAssembleArgs()
-    a.F, a.Fb, a.gamma, a.energy, a.nfields = nat.ptr(s.F), nat.ptr(s.Fb), float(c.viscosity), 0, dim
-    for k in range(dim):
-        a.phi[k], a.grad[k], a.phib[k] = nat.ptr(s.U[k]), nat.ptr(s.gU[k]), nat.ptr(s.ub[k])
-        a.b[k], a.x0[k] = nat.ptr(s.bmom[k]), nat.ptr(s.Ustar[k])
-    a.gp, a.has_drive, a.relax, a.second_order, a.simplec, a.cp = nat.ptr(s.gp), 0, float(s.relax_u), 1, 1, 1.0
-    a.diag, a.diag_r, a.off, a.dP = nat.ptr(s.diag), nat.ptr(s.diag_r), nat.ptr(s.off), nat.ptr(s.dP)
-    a.resid, a.inv_r = nat.ptr(s.scal), nat.ptr(s.inv_r)
-    M, B, R = ctypes.byref(fv.c_mesh), ctypes.byref(s.c_bc), ctypes.byref(s.c_red)
-    st = nat.stream_ptr()
-    for _ in range(3):
-        nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, st), "assemble")
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    L.ftmi_timer_enable(1)
+    for _ in range(steps):
+        s.step()
     torch.cuda.synchronize()
-    e0.record()
-    for _ in range(reps):
-        nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, st), "assemble")
-    e1.record()
-    torch.cuda.synchronize()
-    t = e0.elapsed_time(e1) * 1e-3 / reps
-    nbytes = _assemble_bytes(s)
+    ms, cnt = ctypes.c_double(0.0), ctypes.c_int64(0)
+    L.ftmi_timer_read(ctypes.byref(ms), ctypes.byref(cnt))
+    L.ftmi_timer_enable(0)
+    t = ms.value * 1e-3 / max(cnt.value, 1)
+    nbytes = bi_pv_bytes(s.nc, s.fv.n_if)
     gbs = nbytes / t / 1e9
-    return {"kernel": "k_assemble<2,2> (fused momentum assembly)", "bound": "hbm", "achieved": round(gbs, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": nbytes, "us_per_launch": round(t * 1e6, 2)}
+    traffic = None
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_duct1M.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            traffic = json.load(fh).get("bi_pv_hbm_bytes_per_launch")
+    return {"kernel": "bi_pv<LduOp<double>> (BiCGSTAB p-update fused with the face-operator SpMV)",
+            "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": nbytes,
+            "us_per_launch": round(t * 1e6, 2), "launches_timed": int(cnt.value)}
+
+
+def calibration_copy(nbytes=256 << 20):
+    """One device-to-device copy of known size: the reference dispatch against which the
+    FETCH_SIZE / WRITE_SIZE counters of a profiled run are scaled."""
+    import torch
+
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    return nbytes
 
 
 def cpu_baseline(nx, ny, iters):
@@ -129,7 +132,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=1000)
     ap.add_argument("--ny", type=int, default=500)
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -160,6 +163,7 @@ def main():
     print(f"[bench] timed {args.steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
           f"{s.amg_setup_seconds:.3f} s; last {s.last_linear}", file=sys.stderr, flush=True)
     roof = measure_roofline(s)
+    calibration_copy()
     out = {"metric": "SIMPLEC outer it/s at 1M cells (FP64)", "value": round(args.steps / dt, 3), "unit": "it/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",

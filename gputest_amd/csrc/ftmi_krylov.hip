@@ -77,6 +77,43 @@ static int post_state(Pinned* pn, int slot, const State* d, hipStream_t s) {
 }
 static int wait_state(Pinned* pn, int slot) { return hipEventSynchronize(pn->ev[slot]) == hipSuccess ? 0 : -1; }
 
+// ------------------------------------------------------------------ opt-in kernel timer
+// When enabled, the launches of one kernel (the BiCGSTAB p-update + SpMV, the solver's
+// largest share of time) are bracketed by HIP events on their own stream, so a benchmark
+// can read that kernel's average duration inside a real solve.
+struct KTimer {
+    bool on = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms = 0.0;
+    int64_t launches = 0;
+    hipEvent_t get() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        // device-scope release: the event does not write back and invalidate caches
+        // system-wide, so the bracket measures the kernel rather than a cache flush
+        hipEvent_t e = nullptr;
+        hipEventCreateWithFlags(&e, hipEventReleaseToDevice);
+        return e;
+    }
+    void drain() {
+        for (auto& pr : pending) {
+            float ms = 0.f;
+            if (hipEventSynchronize(pr.second) == hipSuccess && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+                total_ms += ms;
+                ++launches;
+            }
+            pool.push_back(pr.first);
+            pool.push_back(pr.second);
+        }
+        pending.clear();
+    }
+};
+static KTimer g_timer;
+
 // ------------------------------------------------------------------ BiCGSTAB kernels
 // One iteration is four launches: p-update fused into v = A phat (phat recomputed at the
 // neighbours from r, p, v of the previous iteration), s-update fused into t = A shat, the
@@ -350,7 +387,17 @@ int bicgstab_run(int64_t n, Op A, const double* b, const double* inv, double* x,
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (g_timer.on) {
+            t0 = g_timer.get();
+            t1 = g_timer.get();
+            hipEventRecord(t0, st);
+        }
         bi_pv<Op><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S, W.part[0]);
+        if (g_timer.on) {
+            hipEventRecord(t1, st);
+            g_timer.pending.emplace_back(t0, t1);
+        }
         bi_st<Op><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1], (int)k);
         bi_r<<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0], tol,
                                    (int)k);
@@ -390,6 +437,21 @@ template int bicgstab_run<LduOp<double>>(int64_t, LduOp<double>, const double*, 
 }  // namespace ftmi
 
 extern "C" {
+
+int ftmi_timer_enable(int on) {
+    g_timer.drain();
+    g_timer.on = on != 0;
+    g_timer.total_ms = 0.0;
+    g_timer.launches = 0;
+    return 0;
+}
+
+int ftmi_timer_read(double* total_ms, int64_t* launches) {
+    g_timer.drain();
+    *total_ms = g_timer.total_ms;
+    *launches = g_timer.launches;
+    return 0;
+}
 
 int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data,
                          const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,

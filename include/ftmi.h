@@ -83,6 +83,11 @@ int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indice
                          const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
                          int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
 
+/* new: opt-in timing of the BiCGSTAB p-update + SpMV kernel (HIP events around each launch on
+ * its stream); enable resets the totals, read waits for the recorded events */
+int ftmi_timer_enable(int on);
+int ftmi_timer_read(double* total_ms, int64_t* launches);
+
 /* ---------------------------------------------------------------- Krylov on the LDU form */
 
 /* new (the reference's momentum/energy solve, simple.py:489-497, on the face layout) */
