@@ -154,14 +154,19 @@ def main():
     for k in s.timers:
         s.timers[k] = 0.0
     s.amg_setup_seconds = 0.0
+    s.amg_rebuilds = 0
+    guard0 = len(s.guard_events)
+    pits = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
         s.step()
+        pits.append(s.last_linear.get("pressure_iters"))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     print(f"[bench] timed {args.steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
-          f"{s.amg_setup_seconds:.3f} s; last {s.last_linear}", file=sys.stderr, flush=True)
+          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds; guard events {len(s.guard_events) - guard0}; "
+          f"pressure iterations {pits}; last {s.last_linear}", file=sys.stderr, flush=True)
     roof = measure_roofline(s)
     calibration_copy()
     out = {"metric": "SIMPLEC outer it/s at 1M cells (FP64)", "value": round(args.steps / dt, 3), "unit": "it/s",
@@ -172,7 +177,8 @@ def main():
                                   f"FP64 SIMPLEC + energy, SA-AMG-PCG tol 0.1",
                       "cells": s.nc, "interior_faces": s.fv.n_if, "precision": "fp64",
                       "stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
-                      "amg_setup_seconds": round(s.amg_setup_seconds, 3)},
+                      "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
+                      "pressure_iters_total": int(sum(p or 0 for p in pits))},
            "roofline": roof}
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)

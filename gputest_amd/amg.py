@@ -23,6 +23,7 @@ import torch
 from . import _native as nat
 
 _RNG_CACHE = {}
+_DEV_RNG_CACHE = {}
 
 
 def start_vector(n: int) -> np.ndarray:
@@ -33,6 +34,22 @@ def start_vector(n: int) -> np.ndarray:
         if len(_RNG_CACHE) < 64:
             _RNG_CACHE[n] = v
     return v
+
+
+def start_vector_device(n: int, device) -> torch.Tensor:
+    """A fresh device copy of ``start_vector(n)`` (the power iteration overwrites it); the
+    uploaded vector is kept per (size, device), so a rebuild copies it within HBM."""
+    key = (n, str(device))
+    v = _DEV_RNG_CACHE.get(key)
+    if v is None:
+        v = torch.as_tensor(start_vector(n), device=device)
+        if len(_DEV_RNG_CACHE) < 64:
+            _DEV_RNG_CACHE[key] = v
+    return v.clone()
+
+
+def _pinned(n: int, dtype) -> torch.Tensor:
+    return torch.empty(n, dtype=dtype, pin_memory=True)
 
 
 class DevCSR:
@@ -112,16 +129,19 @@ class SmoothedAggregationAMG:
     def _spgemm(self, X: DevCSR, Y: DevCSR, xscale=None, agg=None, tval=None, omega_num=None, omega_scale=0.0):
         L, st, dev = nat.lib(), nat.stream_ptr(), self.device
         n = X.n
-        cnt = torch.zeros(n, dtype=torch.int32, device=dev)
+        cnt = torch.zeros(n, dtype=torch.int32, device=dev)  # rows over the hash capacity stay 0
+        long_rows = torch.empty(n, dtype=torch.int32, device=dev)
+        n_long = torch.zeros(1, dtype=torch.int32, device=dev)
         args = (n, nat.ptr(X.ptr), nat.ptr(X.col), nat.ptr(X.val), nat.ptr(Y.ptr), nat.ptr(Y.col), nat.ptr(Y.val),
                 nat.ptr(xscale), nat.ptr(agg), nat.ptr(tval), nat.ptr(omega_num), float(omega_scale))
-        nat.check(L.ftmi_amgs_spgemm(0, *args, nat.ptr(cnt), None, None, None, nat.ptr(self._ovf), st), "spgemm")
+        work = (nat.ptr(self._ovf), nat.ptr(long_rows), nat.ptr(n_long), st)
+        nat.check(L.ftmi_amgs_spgemm(0, *args, nat.ptr(cnt), None, None, None, *work), "spgemm")
         ptr = self._scan(cnt)
         nnz = int(ptr[n].item())
         col = torch.empty(nnz, dtype=torch.int32, device=dev)
         val = torch.empty(nnz, dtype=torch.float64, device=dev)
-        nat.check(L.ftmi_amgs_spgemm(1, *args, nat.ptr(cnt), nat.ptr(ptr), nat.ptr(col), nat.ptr(val),
-                                     nat.ptr(self._ovf), st), "spgemm")
+        nat.check(L.ftmi_amgs_spgemm(1, *args, nat.ptr(cnt), nat.ptr(ptr), nat.ptr(col), nat.ptr(val), *work),
+                  "spgemm")
         if self.debug:
             o = self._ovf.cpu().numpy()
             print(f"[amg] spgemm rows {n} nnz {nnz} overflow {o.tolist()}", flush=True)
@@ -162,32 +182,44 @@ class SmoothedAggregationAMG:
             mask = torch.empty(A.nnz, dtype=torch.uint8, device=dev)
             nat.check(L.ftmi_amgs_strength(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
                                            float(self.theta), nat.ptr(mask), st), "strength")
+            # the strength mask (and a coarse level's pattern) go to pinned host memory while
+            # the GPU runs the power iteration, which does not depend on the aggregates
+            mask_p = _pinned(A.nnz, torch.uint8)
+            mask_p.copy_(mask, non_blocking=True)
             if pattern is None:
-                pattern = (A.ptr.cpu().numpy(), A.col.cpu().numpy())
+                ptr_p, col_p = _pinned(n + 1, torch.int32), _pinned(A.nnz, torch.int32)
+                ptr_p.copy_(A.ptr, non_blocking=True)
+                col_p.copy_(A.col, non_blocking=True)
+                pattern = (ptr_p.numpy(), col_p.numpy())
+            copied = torch.cuda.Event()
+            copied.record()
+            x = start_vector_device(n, dev)
+            y, rho = f64(n), f64(2)
+            nat.check(L.ftmi_amgs_power(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv), nat.ptr(x),
+                                        nat.ptr(y), 15, nat.ptr(partials), nat.ptr(rho), st), "power")
+            copied.synchronize()
             ptr_h, col_h = pattern
-            mask_h = mask.cpu().numpy()
-            agg_h = np.empty(n, np.int32)
-            size_h = np.empty(n, np.int32)
+            mask_h = mask_p.numpy()
+            agg_p, size_p = _pinned(n, torch.int32), _pinned(n, torch.int32)
+            agg_h, size_h = agg_p.numpy(), size_p.numpy()
             na = int(L.ftmi_amg_aggregate_mask_host(n, ptr_h.ctypes.data, col_h.ctypes.data, mask_h.ctypes.data,
                                                     agg_h.ctypes.data, size_h.ctypes.data))
             if na >= n * 0.9:
                 break
-            agg = torch.as_tensor(agg_h, device=dev)
-            size = torch.as_tensor(size_h[:na].copy(), device=dev)
+            agg = torch.empty(n, dtype=torch.int32, device=dev)
+            agg.copy_(agg_p, non_blocking=True)
+            size = torch.empty(na, dtype=torch.int32, device=dev)
+            size.copy_(size_p[:na], non_blocking=True)
             t_ptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
             t_val = f64(n)
             nat.check(L.ftmi_amgs_tentative(n, nat.ptr(agg), nat.ptr(size), nat.ptr(t_ptr), nat.ptr(t_val), st),
                       "tentative")
             T = DevCSR(n, na, t_ptr, agg, t_val)
-            x = torch.as_tensor(start_vector(n), device=dev)
-            y, rho = f64(n), f64(2)
-            nat.check(L.ftmi_amgs_power(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv), nat.ptr(x),
-                                        nat.ptr(y), 15, nat.ptr(partials), nat.ptr(rho), st), "power")
             P = self._spgemm(A, T, xscale=dinv, agg=agg, tval=t_val, omega_num=rho, omega_scale=4.0 / 3.0)
             R = self._transpose(P)
             Ac = self._spgemm(R, self._spgemm(A, P))
             self.levels.append((A, P, R, dinv))
-            self.aggregates.append(agg_h.astype(np.int64))
+            self.aggregates.append(agg_h.astype(np.int64))  # a copy: the pinned buffer is recycled
             self.rhos.append(rho)
             A, pattern = Ac, None
         ovf = self._ovf.cpu().numpy()

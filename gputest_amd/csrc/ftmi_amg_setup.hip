@@ -7,12 +7,14 @@
 // aggregation itself (three greedy passes whose result depends on the node order) runs on
 // the host over the strength mask copied back (ftmi_amg_aggregate_mask_host).
 //
-// Sparse products are computed one wavefront per output row: the lanes insert the row's
-// candidate columns into an LDS hash table (integer compare-and-swap, so the set is
-// deterministic), the distinct columns are ranked into sorted order, and each column's value
-// is summed by one lane in the fixed order of the left operand's row. Results are therefore
-// independent of the schedule. Rows are sized in a count pass, the caller scans the counts
-// into row pointers and allocates, and a fill pass writes the entries.
+// Sparse products run one thread per output row as a k-way merge of the right operand's rows
+// (rows of the left operand up to 32 entries); longer rows go to one wavefront each, whose
+// lanes insert the row's candidate columns into an LDS hash table (integer compare-and-swap,
+// so the set is deterministic) and rank the distinct columns into sorted order. Either way
+// each column's value is summed in the fixed order of the left operand's row, so results are
+// independent of the schedule and equal a row-by-row sparse product. Rows are sized in a
+// count pass, the caller scans the counts into row pointers and allocates, and a fill pass
+// writes the entries.
 #include <cstdio>
 #include <cstring>
 
@@ -147,11 +149,13 @@ __device__ inline int32_t find_in_row(const CsrV& Y, int32_t k, int32_t c) {
     return -1;
 }
 
-// C = Xs Y row by row, Xs = diag(xscale) X when xscale != nullptr. In "prolongator" mode
-// (agg != nullptr) the row is finished as C_rc = [c == agg[r]] t[r] - omega C_rc, the
+// C = Xs Y for the rows listed in rows[0 .. *n_rows) (the rows too long for the merge kernel
+// below), one wavefront per row, Xs = diag(xscale) X when xscale != nullptr. In "prolongator"
+// mode (agg != nullptr) the row is finished as C_rc = [c == agg[r]] t[r] - omega C_rc, the
 // smoothed prolongator T - omega D^-1 A T with Y = T.
 template <bool FILL>
-__global__ void __launch_bounds__(kWave) k_spgemm(int64_t nrows, CsrV X, CsrV Y, const double* __restrict__ xscale,
+__global__ void __launch_bounds__(kWave) k_spgemm(const int32_t* __restrict__ rows, const int32_t* n_rows, CsrV X,
+                                                  CsrV Y, const double* __restrict__ xscale,
                                                   const int32_t* __restrict__ agg, const double* __restrict__ tval,
                                                   const double* omega_num, double omega_scale,
                                                   int32_t* __restrict__ rowcnt, const int32_t* __restrict__ cptr,
@@ -161,7 +165,9 @@ __global__ void __launch_bounds__(kWave) k_spgemm(int64_t nrows, CsrV X, CsrV Y,
     __shared__ int32_t dense[kHash];
     __shared__ int32_t nk, nd;
     const int lane = threadIdx.x;
-    for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int64_t nlist = *n_rows;
+    for (int64_t li = blockIdx.x; li < nlist; li += gridDim.x) {
+        const int64_t r = rows[li];
         const int32_t xb = X.ptr[r], xe = X.ptr[r + 1];
         // table size: a power of two >= twice this row's candidate count (at least 64 slots)
         int bound = 0;
@@ -241,6 +247,81 @@ __global__ void __launch_bounds__(kWave) k_spgemm(int64_t nrows, CsrV X, CsrV Y,
             cval[base + t] = v;
         }
         __syncthreads();
+    }
+}
+
+// C = Xs Y one thread per row, for rows of X with at most kMergeCap entries: a k-way merge of
+// the (column-sorted) rows of Y selected by the row of X. Each step takes the smallest head
+// column, adds the contributions of the heads at that column in the order of X's row (the
+// order a row-by-row sparse product accumulates in) and advances them, so the row comes out
+// sorted with no hash and no LDS. Longer rows are appended to long_rows in the count pass and
+// handled by k_spgemm.
+constexpr int kMergeCap = 32;
+
+template <bool FILL>
+__global__ void __launch_bounds__(kBlock) k_spgemm_merge(int64_t nrows, CsrV X, CsrV Y,
+                                                         const double* __restrict__ xscale,
+                                                         const int32_t* __restrict__ agg,
+                                                         const double* __restrict__ tval, const double* omega_num,
+                                                         double omega_scale, int32_t* __restrict__ rowcnt,
+                                                         const int32_t* __restrict__ cptr,
+                                                         int32_t* __restrict__ ccol, double* __restrict__ cval,
+                                                         int32_t* __restrict__ long_rows, int32_t* n_long) {
+    constexpr int32_t kNone = 0x7fffffff;
+    int64_t lo, hi;
+    block_range(nrows, lo, hi);
+    for (int64_t r = lo + threadIdx.x; r < hi; r += kBlock) {
+        const int32_t xb = X.ptr[r];
+        const int32_t L = X.ptr[r + 1] - xb;
+        if (L > kMergeCap) {
+            if (!FILL) long_rows[atomicAdd(n_long, 1)] = (int32_t)r;
+            continue;
+        }
+        int32_t q[kMergeCap], e[kMergeCap], c[kMergeCap];
+        int32_t cur = kNone;
+#pragma unroll
+        for (int t = 0; t < kMergeCap; ++t) {
+            q[t] = 0;
+            e[t] = 0;
+            c[t] = kNone;
+            if (t < L) {
+                const int32_t k = X.col[xb + t];
+                q[t] = Y.ptr[k];
+                e[t] = Y.ptr[k + 1];
+                if (q[t] < e[t]) c[t] = Y.col[q[t]];
+            }
+            cur = c[t] < cur ? c[t] : cur;
+        }
+        const double xs = (FILL && xscale) ? xscale[r] : 1.0;
+        const int32_t ar = (FILL && agg) ? agg[r] : -1;
+        const double tr = (FILL && agg) ? tval[r] : 0.0;
+        const double om = (FILL && agg) ? omega_scale / *omega_num : 0.0;
+        const int32_t base = FILL ? cptr[r] : 0;
+        int32_t cnt = 0;
+        while (cur != kNone) {
+            double v = 0.0;
+            int32_t nxt = kNone;
+#pragma unroll
+            for (int t = 0; t < kMergeCap; ++t) {
+                if (c[t] == cur) {
+                    if (FILL) {
+                        const double xv = X.val[xb + t];
+                        v += (xscale ? xs * xv : xv) * Y.val[q[t]];
+                    }
+                    ++q[t];
+                    c[t] = q[t] < e[t] ? Y.col[q[t]] : kNone;
+                }
+                nxt = c[t] < nxt ? c[t] : nxt;
+            }
+            if (FILL) {
+                if (agg) v = (cur == ar ? tr : 0.0) - om * v;
+                ccol[base + cnt] = cur;
+                cval[base + cnt] = v;
+            }
+            ++cnt;
+            cur = nxt;
+        }
+        if (!FILL) rowcnt[r] = cnt;
     }
 }
 
@@ -326,18 +407,19 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(int64_t n, const int32_t*
                                                        const int64_t* __restrict__ bsum, int32_t* __restrict__ out) {
     __shared__ long long base;
     __shared__ long long part[kBlock];
-    if (threadIdx.x == 0) {
-        // the XCD-permuted chunk order: chunks are laid out by logical index, so sum the
-        // block sums of every block whose logical index is smaller
-        const int g = gridDim.x, b = blockIdx.x;
+    {
+        // the XCD-permuted chunk order: chunks are laid out by logical index, so the base is
+        // the sum of the block sums of every block whose logical index is smaller
+        const int g = gridDim.x;
         auto logical_of = [g](int bb) { return (g >= kNXcd && g % kNXcd == 0) ? (bb % kNXcd) * (g / kNXcd) + bb / kNXcd : bb; };
-        const int me = logical_of(b);
-        long long s = 0;
-        for (int bb = 0; bb < g; ++bb)
-            if (logical_of(bb) < me) s += bsum[bb];
-        base = s;
+        const int me = logical_of(blockIdx.x);
+        long long acc[1] = {0};
+        for (int bb = threadIdx.x; bb < g; bb += kBlock)
+            if (logical_of(bb) < me) acc[0] += bsum[bb];
+        block_sum<1>(acc);
+        if (threadIdx.x == 0) base = acc[0];
+        __syncthreads();
     }
-    __syncthreads();
     int64_t lo, hi;
     block_range(n, lo, hi);
     long long run = base;
@@ -416,18 +498,27 @@ int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_
                      const int32_t* y_ptr, const int32_t* y_col, const double* y_val, const double* xscale,
                      const int32_t* agg, const double* tval, const double* omega_num, double omega_scale,
                      int32_t* rowcnt, const int32_t* c_ptr, int32_t* c_col, double* c_val, int* overflow,
-                     void* stream) {
+                     int32_t* long_rows, int32_t* n_long, void* stream) {
     if (nrows <= 0) return 0;
+    if (!rowcnt || !long_rows || !n_long || (fill && (!c_ptr || !c_col || !c_val))) {
+        set_error_msg("ftmi_amgs_spgemm: missing work or output buffer");
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)stream;
     CsrV X{x_ptr, x_col, x_val}, Y{y_ptr, y_col, y_val};
-    int64_t g = nrows < 65536 ? nrows : 65536;
-    if (fill)
-        k_spgemm<true><<<(int)g, kWave, 0, (hipStream_t)stream>>>(nrows, X, Y, xscale, agg, tval, omega_num,
-                                                                   omega_scale, rowcnt, c_ptr, c_col, c_val,
-                                                                   overflow);
-    else
-        k_spgemm<false><<<(int)g, kWave, 0, (hipStream_t)stream>>>(nrows, X, Y, xscale, agg, tval, omega_num,
-                                                                    omega_scale, rowcnt, c_ptr, c_col, c_val,
-                                                                    overflow);
+    const int g = grid_for(nrows);
+    const int gl = nrows < kMaxGrid ? (int)nrows : kMaxGrid;
+    if (fill) {
+        k_spgemm_merge<true><<<g, kBlock, 0, s>>>(nrows, X, Y, xscale, agg, tval, omega_num, omega_scale, rowcnt,
+                                                  c_ptr, c_col, c_val, long_rows, n_long);
+        k_spgemm<true><<<gl, kWave, 0, s>>>(long_rows, n_long, X, Y, xscale, agg, tval, omega_num, omega_scale,
+                                            rowcnt, c_ptr, c_col, c_val, overflow);
+    } else {
+        k_spgemm_merge<false><<<g, kBlock, 0, s>>>(nrows, X, Y, xscale, agg, tval, omega_num, omega_scale, rowcnt,
+                                                   c_ptr, c_col, c_val, long_rows, n_long);
+        k_spgemm<false><<<gl, kWave, 0, s>>>(long_rows, n_long, X, Y, xscale, agg, tval, omega_num, omega_scale,
+                                             rowcnt, c_ptr, c_col, c_val, overflow);
+    }
     FTMI_LAUNCH_CHECK("ftmi_amgs_spgemm");
     return 0;
 }

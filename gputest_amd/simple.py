@@ -87,6 +87,7 @@ class SimpleSolver:
         self._amg_age = 10 ** 9
         self.amg_rebuild = 10
         self.amg_setup_seconds = 0.0
+        self.amg_rebuilds = 0
         self.last_linear = {}
         self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
         self._build_boundaries()
@@ -258,6 +259,7 @@ class SimpleSolver:
             self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
+            self.amg_rebuilds += 1
         self._amg_age += 1
         pinfo = pcg_ldu(fv, self.diag_p, self.offp, self.rhs_p, self.pp, c.pressure_tol, 200, self._amg,
                         self.singular_p, ws=self.ws)

@@ -145,12 +145,15 @@ int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, 
 int ftmi_amgs_power(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* dinv,
                     double* x, double* y, int iters, double* partials, double* out, void* stream);
 /* deterministic sparse product C = diag(xscale) X Y, count (fill = 0) or fill pass; with agg:
- * C = T - (omega_scale / *omega_num) diag(xscale) X Y (the smoothed prolongator) */
+ * C = T - (omega_scale / *omega_num) diag(xscale) X Y (the smoothed prolongator). Columns come
+ * out sorted and each entry is summed in the order of X's row. long_rows (nrows int32) and
+ * n_long (1 int32, zero before the count pass) are work shared by the two passes: the count
+ * pass lists the rows of X longer than 32 entries there, and both passes read that list. */
 int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_t* x_col, const double* x_val,
                      const int32_t* y_ptr, const int32_t* y_col, const double* y_val, const double* xscale,
                      const int32_t* agg, const double* tval, const double* omega_num, double omega_scale,
                      int32_t* rowcnt, const int32_t* c_ptr, int32_t* c_col, double* c_val, int* overflow,
-                     void* stream);
+                     int32_t* long_rows, int32_t* n_long, void* stream);
 /* exclusive scan: out[0..n] (out[n] = total); bsum: 2048 int64 of work */
 int ftmi_scan_i32(int64_t n, const int32_t* in, int32_t* out, int64_t* bsum, void* stream);
 /* transpose: column counts, then placement and per-row sort */
