@@ -6,6 +6,10 @@ Re = 500, energy on (inlet T = 300, wall heat flux 5000, Pr = 0.7), FP64 SIMPLEC
 SA-AMG-PCG (pressure tolerance 0.1) and Jacobi-BiCGSTAB. A step is one outer iteration; the
 AMG hierarchy is rebuilt every 10 iterations inside the timed region, as the solver does.
 
+``value`` is the FP64 solver's rate; ``mixed`` times the mixed-precision solver (FP32 transport
+assembly and BiCGSTAB corrections, FP32 AMG-PCG refined in FP64) over the same warm-up and
+timed window from the same initial state.
+
 Prints one JSON line (see ``--help``). ``roofline`` is measured live for the dominant kernel,
 the BiCGSTAB p-update fused with the face-operator SpMV (its launches in three further outer
 iterations bracketed by HIP events on the solver's stream); ``cpu_baseline`` times the CPU
@@ -26,7 +30,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def _workload(nx, ny):
+def _workload(nx, ny, precision="fp64"):
     from gputest_amd import case as C
     from gputest_amd.mesh import duct_mesh
 
@@ -43,7 +47,7 @@ def _workload(nx, ny):
                     "bottom": B("wall", thermal="flux", value=5000.0),
                     "top": B("wall", thermal="flux", value=5000.0)},
         viscosity=mu, energy=True, conductivity=mu / 0.7, cp=1.0, init_T=300.0, pressure_tol=0.1,
-        max_iter=10 ** 9, tol=0.0)
+        max_iter=10 ** 9, tol=0.0, precision=precision)
     return duct_mesh(nx, ny), case, parabolic
 
 
@@ -125,6 +129,40 @@ def cpu_baseline(nx, ny, iters):
                       f"(AMG rebuilt every 10 iterations; NumPy/SciPy, single-threaded)"}
 
 
+def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
+    """Warm-up, then `steps` timed outer iterations bracketed by device synchronisation."""
+    import torch
+
+    t0 = time.perf_counter()
+    s = SimpleSolver(mesh, case, "cuda")
+    print(f"[bench:{tag}] set-up {time.perf_counter() - t0:.1f} s: {s.nc} cells, {s.fv.n_if} interior faces",
+          file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        s.step()
+    print(f"[bench:{tag}] warm-up done: {s.last_linear}", file=sys.stderr, flush=True)
+    for k in s.timers:
+        s.timers[k] = 0.0
+    s.amg_setup_seconds = 0.0
+    s.amg_rebuilds = 0
+    guard0 = len(s.guard_events)
+    pits = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+        pits.append(s.last_linear.get("pressure_iters"))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    guards = len(s.guard_events) - guard0
+    print(f"[bench:{tag}] timed {steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
+          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds; guard events {guards}; "
+          f"pressure iterations {pits}; last {s.last_linear}", file=sys.stderr, flush=True)
+    info = {"stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
+            "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
+            "guard_events": guards, "pressure_iters_total": int(sum(p or 0 for p in pits))}
+    return s, dt, info
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +172,7 @@ def main():
     ap.add_argument("--ny", type=int, default=500)
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision solver (1) or not (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SIMPLEC path on a single MI355X "
@@ -144,29 +183,14 @@ def main():
     from gputest_amd.simple import SimpleSolver
 
     mesh, case, _ = _workload(args.nx, args.ny)
-    t0 = time.perf_counter()
-    s = SimpleSolver(mesh, case, "cuda")
-    setup = time.perf_counter() - t0
-    print(f"[bench] set-up {setup:.1f} s: {s.nc} cells, {s.fv.n_if} interior faces", file=sys.stderr, flush=True)
-    for i in range(args.warmup):
-        s.step()
-    print(f"[bench] warm-up done: {s.last_linear}", file=sys.stderr, flush=True)
-    for k in s.timers:
-        s.timers[k] = 0.0
-    s.amg_setup_seconds = 0.0
-    s.amg_rebuilds = 0
-    guard0 = len(s.guard_events)
-    pits = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        s.step()
-        pits.append(s.last_linear.get("pressure_iters"))
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    print(f"[bench] timed {args.steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
-          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds; guard events {len(s.guard_events) - guard0}; "
-          f"pressure iterations {pits}; last {s.last_linear}", file=sys.stderr, flush=True)
+    s, dt, info = run_timed(SimpleSolver, mesh, case, args.warmup, args.steps, "fp64")
+    mixed = None
+    if args.mixed:
+        _, mcase, _ = _workload(args.nx, args.ny, "mixed")
+        sm, dtm, infom = run_timed(SimpleSolver, mesh, mcase, args.warmup, args.steps, "mixed")
+        mixed = {"value": round(args.steps / dtm, 3), "ms_per_step": round(dtm / args.steps * 1e3, 3),
+                 "speedup_vs_fp64": round(dt / dtm, 3), "dtype": "f32 solves + f64 refinement", **infom}
+        del sm
     roof = measure_roofline(s)
     calibration_copy()
     out = {"metric": "SIMPLEC outer it/s at 1M cells (FP64)", "value": round(args.steps / dt, 3), "unit": "it/s",
@@ -176,10 +200,10 @@ def main():
            "config": {"workload": f"heated duct {args.nx}x{args.ny}x2 jittered triangles ({s.nc} cells), "
                                   f"FP64 SIMPLEC + energy, SA-AMG-PCG tol 0.1",
                       "cells": s.nc, "interior_faces": s.fv.n_if, "precision": "fp64",
-                      "stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
-                      "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
-                      "pressure_iters_total": int(sum(p or 0 for p in pits))},
+                      **info},
            "roofline": roof}
+    if mixed is not None:
+        out["mixed"] = mixed
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_iters)

@@ -40,7 +40,8 @@ class FtmiAssembleArgs(ctypes.Structure):
                 ("drive", F64 * 3), ("has_drive", I32), ("relax", F64), ("second_order", I32),
                 ("simplec", I32), ("cp", F64),
                 ("diag", P), ("diag_r", P), ("off", P), ("b", P * 3), ("dP", P), ("resid", P),
-                ("inv_r", P), ("x0", P * 3)]
+                ("inv_r", P), ("x0", P * 3), ("out_fp32", I32),
+                ("diag32", P), ("diag_r32", P), ("off32", P), ("b32", P * 3), ("inv_r32", P), ("ex32", P)]
 
 
 _SIGS = {
@@ -66,6 +67,14 @@ _SIGS = {
     "ftmi_amg_add_level": ([P, I64] + [P] * 14, I32),
     "ftmi_amg_set_coarse": ([P, I64, P, P, P], I32),
     "ftmi_amg_vcycle": ([P, P, P, P], I32),
+    "ftmi_amg_create_f32": ([F64, I32], P),
+    "ftmi_amg_add_level_f32": ([P, I64] + [P] * 14, I32),
+    "ftmi_amg_set_coarse_f32": ([P, I64, P, P, P], I32),
+    "ftmi_amg_vcycle_f32": ([P, P, P, P], I32),
+    "ftmi_pcg_ldu_f32": ([I64, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
+    "ftmi_bicgstab_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
+    "ftmi_pcg_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, P, I32, I32, P, P, P, P, P], I32),
+    "ftmi_cast_f64_f32": ([I64, P, P, P], I32),
     "ftmi_amg_aggregate_host": ([I64, P, P, P], I64),
     "ftmi_amg_aggregate_mask_host": ([I64, P, P, P, P, P], I64),
     "ftmi_amgs_ldu_to_csr": ([I64] + [P] * 8, I32),

@@ -256,6 +256,10 @@ class SmoothedAggregationAMG:
         nat.check(L.ftmi_amg_set_coarse(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb), nat.ptr(self._cx)),
                   "amg_set_coarse")
 
+    def to_fp32(self) -> "AmgFP32":
+        """The same hierarchy with FP32 values and vectors (the mixed-precision pressure solve)."""
+        return AmgFP32(self)
+
     # ------------------------------------------------------------------ use
     @property
     def n_levels(self) -> int:
@@ -272,6 +276,49 @@ class SmoothedAggregationAMG:
         b = b.contiguous()
         x = torch.empty_like(b)
         nat.check(nat.lib().ftmi_amg_vcycle(self.handle, nat.ptr(b), nat.ptr(x), nat.stream_ptr()), "amg_vcycle")
+        return x
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            try:
+                nat.lib().ftmi_amg_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+class AmgFP32:
+    """FP32 twin of a set-up ``SmoothedAggregationAMG``: the patterns are shared, the values
+    (A, P, R, D^-1, the coarse inverse) are rounded once to FP32, and ``amg(b)`` applies one
+    V-cycle in FP32 arithmetic. Holds a reference to its FP64 source, whose patterns it uses."""
+
+    def __init__(self, src: SmoothedAggregationAMG):
+        from .krylov import to_f32
+
+        L, dev = nat.lib(), src.device
+        self.src, self.device = src, dev
+        self.handle = L.ftmi_amg_create_f32(float(src.omega), int(src.sweeps))
+        self._keep = []
+        for A, P, R, dinv in src.levels:
+            av, pv, rv, dv = to_f32(A.val), to_f32(P.val), to_f32(R.val), to_f32(dinv)
+            v = [torch.zeros(A.n, dtype=torch.float32, device=dev) for _ in range(4)]
+            self._keep += [av, pv, rv, dv] + v
+            nat.check(L.ftmi_amg_add_level_f32(self.handle, A.n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(av),
+                                               nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(pv), nat.ptr(R.ptr),
+                                               nat.ptr(R.col), nat.ptr(rv), nat.ptr(dv), *[nat.ptr(t) for t in v]),
+                      "amg_add_level_f32")
+        nc = src.coarse_n
+        self._cinv = to_f32(src._cinv)
+        self._cb = torch.zeros(nc, dtype=torch.float32, device=dev)
+        self._cx = torch.zeros(nc, dtype=torch.float32, device=dev)
+        nat.check(L.ftmi_amg_set_coarse_f32(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb),
+                                            nat.ptr(self._cx)), "amg_set_coarse_f32")
+
+    def __call__(self, b: torch.Tensor) -> torch.Tensor:
+        b = b.contiguous()
+        x = torch.empty_like(b)
+        nat.check(nat.lib().ftmi_amg_vcycle_f32(self.handle, nat.ptr(b), nat.ptr(x), nat.stream_ptr()), "vcycle_f32")
         return x
 
     def __del__(self):

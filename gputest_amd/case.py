@@ -79,8 +79,9 @@ class UnstructuredCase:
             raise ValueError("pressure_gradient must be 'green-gauss' or 'least-squares'")
         if self.scheme not in ("upwind", "second-order"):
             raise ValueError("scheme must be 'upwind' or 'second-order'")
-        if self.precision not in ("fp64",):
-            raise ValueError("precision must be 'fp64'")
+        if self.precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed' (FP32 transport assembly and Krylov solves, "
+                             "FP32 AMG-PCG refined in FP64)")
         self.periodic = [list(p) for p in self.periodic]
 
     def to_dict(self) -> dict:

@@ -121,20 +121,26 @@ static KTimer g_timer;
 // between two buffers so that no kernel reads what it writes. The scalars of a reduction are
 // computed by every workgroup of the kernel that consumes it (consume_sum), so only the
 // end-of-iteration state needs its own one-workgroup kernel.
-template <class Op>
-__global__ void __launch_bounds__(kBlock) bi_init(int64_t n, Op A, const double* __restrict__ b,
-                                                  const double* __restrict__ x, double* __restrict__ r,
-                                                  double* __restrict__ rh, double* __restrict__ p,
-                                                  double* __restrict__ v, double* partials) {
+// ZERO: the start vector is zero (x is cleared and r = b without the product)
+template <class Op, typename T, bool ZERO>
+__global__ void __launch_bounds__(kBlock) bi_init(int64_t n, Op A, const T* __restrict__ b, T* __restrict__ x,
+                                                  T* __restrict__ r, T* __restrict__ rh, T* __restrict__ p,
+                                                  T* __restrict__ v, double* partials) {
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[2] = {0.0, 0.0};
+    T acc[2] = {T(0), T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double ri = b[i] - A.row(i, x);
+        T ri;
+        if (ZERO) {
+            x[i] = T(0);
+            ri = b[i];
+        } else {
+            ri = b[i] - A.row(i, x);
+        }
         r[i] = ri;
         rh[i] = ri;
-        p[i] = 0.0;
-        v[i] = 0.0;
+        p[i] = T(0);
+        v[i] = T(0);
         acc[0] += ri * ri;
         acc[1] += b[i] * b[i];
     }
@@ -167,25 +173,25 @@ __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, in
 }
 
 // p = r + beta (p_prev - omega v_prev), phat = inv p, v = A phat; partials of rhat . v
-template <class Op>
-__global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const double* __restrict__ r,
-                                                const double* __restrict__ p_prev, const double* __restrict__ v_prev,
-                                                const double* __restrict__ inv, const double* __restrict__ rh,
-                                                double* __restrict__ p, double* __restrict__ ph,
-                                                double* __restrict__ v, const State* S, double* part_out) {
+template <class Op, typename T>
+__global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const T* __restrict__ r,
+                                                const T* __restrict__ p_prev, const T* __restrict__ v_prev,
+                                                const T* __restrict__ inv, const T* __restrict__ rh,
+                                                T* __restrict__ p, T* __restrict__ ph, T* __restrict__ v,
+                                                const State* S, double* part_out) {
     if (S->st[I_DONE]) return;
     const double* sc = S->sc;
-    const double beta = (sc[S_RHONEW] / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]);
-    const double om = sc[S_OMEGA];
+    const T beta = (T)((sc[S_RHONEW] / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]));
+    const T om = (T)sc[S_OMEGA];
     auto phat = [&](int64_t j) { return inv[j] * (r[j] + beta * (p_prev[j] - om * v_prev[j])); };
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double pi = r[i] + beta * (p_prev[i] - om * v_prev[i]);
+        const T pi = r[i] + beta * (p_prev[i] - om * v_prev[i]);
         p[i] = pi;
         ph[i] = inv[i] * pi;
-        const double vi = A.row_f(i, phat);
+        const T vi = A.row_f(i, phat);
         v[i] = vi;
         acc[0] += rh[i] * vi;
     }
@@ -193,11 +199,10 @@ __global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const double* _
 }
 
 // alpha = rho / (rhat . v); s = r - alpha v, shat = inv s, t = A shat; partials s.s, t.t, t.s
-template <class Op>
-__global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const double* __restrict__ r,
-                                                const double* __restrict__ v, const double* __restrict__ inv,
-                                                double* __restrict__ s, double* __restrict__ sh,
-                                                double* __restrict__ t, State* S, const double* part_in, int np_in,
+template <class Op, typename T>
+__global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const T* __restrict__ r, const T* __restrict__ v,
+                                                const T* __restrict__ inv, T* __restrict__ s, T* __restrict__ sh,
+                                                T* __restrict__ t, State* S, const double* part_in, int np_in,
                                                 double* part_out, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
@@ -214,20 +219,21 @@ __global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const double* _
         }
         return;
     }
-    const double alpha = rho / tot[0];
+    const double alpha_d = rho / tot[0];
     if (lead) {
         S->sc[S_RHO] = rho;
-        S->sc[S_ALPHA] = alpha;
+        S->sc[S_ALPHA] = alpha_d;
     }
+    const T alpha = (T)alpha_d;
     auto shat = [&](int64_t j) { return inv[j] * (r[j] - alpha * v[j]); };
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[3] = {0.0, 0.0, 0.0};
+    T acc[3] = {T(0), T(0), T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double si = r[i] - alpha * v[i];
+        const T si = r[i] - alpha * v[i];
         s[i] = si;
         sh[i] = inv[i] * si;
-        const double ti = A.row_f(i, shat);
+        const T ti = A.row_f(i, shat);
         t[i] = ti;
         acc[0] += si * si;
         acc[1] += ti * ti;
@@ -237,12 +243,12 @@ __global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const double* _
 }
 
 // early exit on ||s||; omega = (t.s)/(t.t); r = s - omega t; x_other = x_cur + alpha phat + omega shat
-__global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const double* __restrict__ s, const double* __restrict__ t,
-                                               const double* __restrict__ ph, const double* __restrict__ sh,
-                                               const double* __restrict__ rh, double* __restrict__ r,
-                                               double* __restrict__ x0, double* __restrict__ x1, State* S,
-                                               const double* part_in, int np_in, double* part_out, double tol,
-                                               int k) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const T* __restrict__ s, const T* __restrict__ t,
+                                               const T* __restrict__ ph, const T* __restrict__ sh,
+                                               const T* __restrict__ rh, T* __restrict__ r, T* __restrict__ x0,
+                                               T* __restrict__ x1, State* S, const double* part_in, int np_in,
+                                               double* part_out, double tol, int k) {
     if (S->st[I_DONE]) return;
     double tot[3];
     consume_sum<3>(part_in, np_in, tot);
@@ -258,17 +264,18 @@ __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const double* __restri
         }
         return;
     }
-    const double om = tot[1] > 0.0 ? tot[2] / tot[1] : 0.0;
-    if (lead) S->sc[S_OMEGA] = om;
-    const double alpha = S->sc[S_ALPHA];
+    const double om_d = tot[1] > 0.0 ? tot[2] / tot[1] : 0.0;
+    if (lead) S->sc[S_OMEGA] = om_d;
+    const T om = (T)om_d;
+    const T alpha = (T)S->sc[S_ALPHA];
     const int cur = S->st[I_CUR];
-    const double* xc = cur ? x1 : x0;
-    double* xo = cur ? x0 : x1;
+    const T* xc = cur ? x1 : x0;
+    T* xo = cur ? x0 : x1;
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[2] = {0.0, 0.0};
+    T acc[2] = {T(0), T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double ri = s[i] - om * t[i];
+        const T ri = s[i] - om * t[i];
         r[i] = ri;
         xo[i] = xc[i] + alpha * ph[i] + om * sh[i];
         acc[0] += ri * ri;
@@ -317,29 +324,39 @@ __global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int n
 }
 
 // x0 = x_cur + alpha phat (early exit on ||s||), or x0 = x1 when x1 is current.
-__global__ void __launch_bounds__(kBlock) bi_final(int64_t n, double* __restrict__ x0, const double* __restrict__ x1,
-                                                   const double* __restrict__ ph, double alpha, int cur, int fixup) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bi_final(int64_t n, T* __restrict__ x0, const T* __restrict__ x1,
+                                                   const T* __restrict__ ph, double alpha_d, int cur, int fixup) {
+    const T alpha = (T)alpha_d;
     int64_t lo, hi;
     block_range(n, lo, hi);
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double xc = cur ? x1[i] : x0[i];
+        const T xc = cur ? x1[i] : x0[i];
         x0[i] = fixup ? xc + alpha * ph[i] : xc;
     }
 }
 
+template <typename T>
 struct BiWork {
-    double *r, *rh, *p[2], *v[2], *ph, *s, *sh, *t, *x1;
+    T *r, *rh, *p[2], *v[2], *ph, *s, *sh, *t, *x1;
     State* S;
     unsigned* counter;
     double* part[2];
 };
 
 constexpr int64_t kPartBytes = (int64_t)kRedGrid * kMaxRed * 8;
+inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
-static BiWork carve_bi(int64_t n, void* work) {
+template <typename T>
+static int64_t bi_bytes(int64_t n) {
+    return align256(11 * n * (int64_t)sizeof(T)) + 512 + 2 * kPartBytes;
+}
+
+template <typename T>
+static BiWork<T> carve_bi(int64_t n, void* work) {
     char* w = (char*)work;
-    BiWork W;
-    double* d = (double*)w;
+    BiWork<T> W;
+    T* d = (T*)w;
     W.r = d;
     W.rh = d + n;
     W.p[0] = d + 2 * n;
@@ -351,7 +368,7 @@ static BiWork carve_bi(int64_t n, void* work) {
     W.sh = d + 8 * n;
     W.t = d + 9 * n;
     W.x1 = d + 10 * n;
-    char* tail = w + ((11 * n * sizeof(double) + 255) / 256) * 256;
+    char* tail = w + align256(11 * n * (int64_t)sizeof(T));
     W.S = (State*)tail;
     W.counter = (unsigned*)(tail + 256);
     W.part[0] = (double*)(tail + 512);
@@ -359,27 +376,36 @@ static BiWork carve_bi(int64_t n, void* work) {
     return W;
 }
 
+// the mixed-precision transport solve keeps its FP32 right-hand side and correction in
+// front of an FP32 BiCGSTAB workspace
+static int64_t bi_mixed_bytes(int64_t n) { return align256(2 * n * (int64_t)sizeof(float)) + bi_bytes<float>(n); }
+
 }  // namespace
 
 extern "C" int64_t ftmi_bicgstab_work_bytes(int64_t n) {
-    return ((11 * n * (int64_t)sizeof(double) + 255) / 256) * 256 + 512 + 2 * kPartBytes;
+    const int64_t a = bi_bytes<double>(n), b = bi_mixed_bytes(n);
+    return a > b ? a : b;
 }
 
 namespace ftmi {
 
 // Runs the whole solve; returns 1 if converged, 0 if not, <0 on a HIP error.
-template <class Op>
-int bicgstab_run(int64_t n, Op A, const double* b, const double* inv, double* x, double tol, int64_t maxiter,
-                 int rel, void* work, int64_t* iters, double* res, hipStream_t st) {
+// zero_x0: x is cleared and taken as the start vector (no product in the initial residual)
+template <class Op, typename T>
+int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, int64_t maxiter, int rel, void* work,
+                 int64_t* iters, double* res, hipStream_t st, bool zero_x0) {
     Pinned* pn = pinned();
     if (!pn) {
         set_error_msg("bicgstab: pinned host memory unavailable");
         return -1;
     }
-    BiWork W = carve_bi(n, work);
+    BiWork<T> W = carve_bi<T>(n, work);
     const int g = grid_red(n);
     k_state_reset<<<1, 64, 0, st>>>(W.S, W.counter);
-    bi_init<Op><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
+    if (zero_x0)
+        bi_init<Op, T, true><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
+    else
+        bi_init<Op, T, false><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
     fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel);
     FTMI_LAUNCH_CHECK("bicgstab init");
     if (post_state(pn, 0, W.S, st)) return -1;
@@ -393,14 +419,16 @@ int bicgstab_run(int64_t n, Op A, const double* b, const double* inv, double* x,
             t1 = g_timer.get();
             hipEventRecord(t0, st);
         }
-        bi_pv<Op><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S, W.part[0]);
+        bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
+                                           W.part[0]);
         if (g_timer.on) {
             hipEventRecord(t1, st);
             g_timer.pending.emplace_back(t0, t1);
         }
-        bi_st<Op><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1], (int)k);
-        bi_r<<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0], tol,
-                                   (int)k);
+        bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
+                                           (int)k);
+        bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
+                                      tol, (int)k);
         fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter);
         FTMI_LAUNCH_CHECK("bicgstab iteration");
         if (post_state(pn, last ^ 1, W.S, st)) return -1;
@@ -424,15 +452,15 @@ int bicgstab_run(int64_t n, Op A, const double* b, const double* inv, double* x,
         conv = 0;
     }
     if (H.st[I_CUR] || H.st[I_FIXUP])
-        bi_final<<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, H.sc[S_ALPHA], H.st[I_CUR], H.st[I_FIXUP]);
+        bi_final<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, H.sc[S_ALPHA], H.st[I_CUR], H.st[I_FIXUP]);
     FTMI_LAUNCH_CHECK("bicgstab final");
     return conv;
 }
 
-template int bicgstab_run<CsrOp<double>>(int64_t, CsrOp<double>, const double*, const double*, double*, double,
-                                         int64_t, int, void*, int64_t*, double*, hipStream_t);
-template int bicgstab_run<LduOp<double>>(int64_t, LduOp<double>, const double*, const double*, double*, double,
-                                         int64_t, int, void*, int64_t*, double*, hipStream_t);
+template int bicgstab_run<CsrOp<double>, double>(int64_t, CsrOp<double>, const double*, const double*, double*,
+                                                 double, int64_t, int, void*, int64_t*, double*, hipStream_t, bool);
+template int bicgstab_run<LduOp<double>, double>(int64_t, LduOp<double>, const double*, const double*, double*,
+                                                 double, int64_t, int, void*, int64_t*, double*, hipStream_t, bool);
 
 }  // namespace ftmi
 
@@ -462,8 +490,8 @@ int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indice
         return 1;
     }
     CsrOp<double> A{indptr, indices, data};
-    return bicgstab_run(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work, iters, res,
-                        (hipStream_t)stream);
+    return bicgstab_run<CsrOp<double>, double>(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work, iters,
+                                               res, (hipStream_t)stream, false);
 }
 
 int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
@@ -475,8 +503,8 @@ int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const i
         return 1;
     }
     LduOp<double> A{diag, ptr, nbr, off};
-    return bicgstab_run(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work, iters, res,
-                        (hipStream_t)stream);
+    return bicgstab_run<LduOp<double>, double>(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work, iters,
+                                               res, (hipStream_t)stream, false);
 }
 
 }  // extern "C"
@@ -484,23 +512,31 @@ int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const i
 // ====================================================================== AMG V-cycle
 namespace {
 
+template <typename T>
 struct Level {
     int64_t n;
-    CsrOp<double> A, P, R;
+    CsrOp<T> A, P, R;
     int tpr_a, tpr_p, tpr_r;  // threads per row of the level's CSR kernels
     int64_t nnz_p;
-    const double* Dinv;
-    double *b, *x, *y, *r;  // level vectors (level 0: b is the caller's rhs, x its output)
+    const T* Dinv;
+    T *b, *x, *y, *r;  // level vectors (level 0: b is the caller's rhs, x its output)
 };
 
-struct Amg {
+// A hierarchy handle holds FP64 or FP32 values; `bits` says which (checked at every entry).
+struct AmgBase {
+    int bits = 64;
+};
+
+template <typename T>
+struct Amg : AmgBase {
     int nlev = 0;
-    Level lv[24];
+    Level<T> lv[24];
     int64_t nc = 0;
-    const double* cinv = nullptr;
-    double *cb = nullptr, *cx = nullptr;
+    const T* cinv = nullptr;
+    T *cb = nullptr, *cx = nullptr;
     double omega = 0.8;
     int sweeps = 2;
+    Amg() { bits = 8 * (int)sizeof(T); }
 };
 
 // Rows of the coarse Galerkin levels grow long (hundreds of entries on the last levels), so
@@ -528,16 +564,16 @@ struct RowGroups {  // row r handled by lanes [g*TPR, g*TPR + TPR) of the grid
     }
 };
 
-template <int TPR>
-__device__ inline double seg_sum(double v) {
+template <int TPR, typename T>
+__device__ inline T seg_sum(T v) {
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, TPR);
     return v;
 }
 
-template <int TPR>
-__device__ inline double row_dot(const CsrOp<double>& A, int64_t i, const double* __restrict__ x, int lane) {
-    double s = 0.0;
+template <int TPR, typename T>
+__device__ inline T row_dot(const CsrOp<T>& A, int64_t i, const T* __restrict__ x, int lane) {
+    T s = T(0);
     const int32_t e = A.ptr[i + 1];
     for (int32_t j = A.ptr[i] + lane; j < e; j += TPR) s += A.val[j] * x[A.col[j]];
     return seg_sum<TPR>(s);
@@ -546,14 +582,15 @@ __device__ inline double row_dot(const CsrOp<double>& A, int64_t i, const double
 inline int grid_rows(int64_t n, int tpr) { return grid_for(n * tpr); }
 
 // x = w D b + w D (b - A (w D b)): the first two pre-smoothing sweeps from x = 0, fused.
-template <int TPR>
-__global__ void __launch_bounds__(kBlock) vc_pre2(int64_t n, CsrOp<double> A, const double* __restrict__ Dinv,
-                                                  const double* __restrict__ b, double* __restrict__ x, double w,
+template <int TPR, typename T>
+__global__ void __launch_bounds__(kBlock) vc_pre2(int64_t n, CsrOp<T> A, const T* __restrict__ Dinv,
+                                                  const T* __restrict__ b, T* __restrict__ x, double w_d,
                                                   const int* done) {
     if (done && *done) return;
+    const T w = (T)w_d;
     RowGroups<TPR> G;
     for (int64_t i = G.first; i < n; i += G.stride) {
-        double ax = 0.0;
+        T ax = T(0);
         const int32_t e = A.ptr[i + 1];
         for (int32_t j = A.ptr[i] + G.lane; j < e; j += TPR) {
             const int32_t c = A.col[j];
@@ -561,35 +598,37 @@ __global__ void __launch_bounds__(kBlock) vc_pre2(int64_t n, CsrOp<double> A, co
         }
         ax = seg_sum<TPR>(ax);
         if (G.lane == 0) {
-            const double wd = w * Dinv[i];
+            const T wd = w * Dinv[i];
             x[i] = wd * b[i] + wd * (b[i] - ax);
         }
     }
 }
 
 // x = w D b (first sweep only; used when sweeps == 1)
-__global__ void __launch_bounds__(kBlock) vc_pre1(int64_t n, const double* __restrict__ Dinv,
-                                                  const double* __restrict__ b, double* __restrict__ x, double w,
-                                                  const int* done) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) vc_pre1(int64_t n, const T* __restrict__ Dinv, const T* __restrict__ b,
+                                                  T* __restrict__ x, double w_d, const int* done) {
     if (done && *done) return;
+    const T w = (T)w_d;
     int64_t lo, hi;
     block_range(n, lo, hi);
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) x[i] = (w * Dinv[i]) * b[i];
 }
 
 // y = x + w D (b - A x)   (optionally also the workgroup partial sums of y)
-template <int TPR, bool SUM>
-__global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<double> A, const double* __restrict__ Dinv,
-                                                    const double* __restrict__ b, const double* __restrict__ x,
-                                                    double* __restrict__ y, double w, const int* done,
+template <int TPR, bool SUM, typename T>
+__global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<T> A, const T* __restrict__ Dinv,
+                                                    const T* __restrict__ b, const T* __restrict__ x,
+                                                    T* __restrict__ y, double w_d, const int* done,
                                                     double* sum_partials) {
     if (done && *done) return;
+    const T w = (T)w_d;
     RowGroups<TPR> G;
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = G.first; i < n; i += G.stride) {
-        const double ax = row_dot<TPR>(A, i, x, G.lane);
+        const T ax = row_dot<TPR>(A, i, x, G.lane);
         if (G.lane == 0) {
-            const double yi = x[i] + (w * Dinv[i]) * (b[i] - ax);
+            const T yi = x[i] + (w * Dinv[i]) * (b[i] - ax);
             y[i] = yi;
             if (SUM) acc[0] += yi;
         }
@@ -597,41 +636,41 @@ __global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<double> A, 
     if (SUM) block_partials<1>(acc, sum_partials);
 }
 
-template <int TPR>
-__global__ void __launch_bounds__(kBlock) vc_resid(int64_t n, CsrOp<double> A, const double* __restrict__ b,
-                                                   const double* __restrict__ x, double* __restrict__ r,
-                                                   const int* done) {
+template <int TPR, typename T>
+__global__ void __launch_bounds__(kBlock) vc_resid(int64_t n, CsrOp<T> A, const T* __restrict__ b,
+                                                   const T* __restrict__ x, T* __restrict__ r, const int* done) {
     if (done && *done) return;
     RowGroups<TPR> G;
     for (int64_t i = G.first; i < n; i += G.stride) {
-        const double ax = row_dot<TPR>(A, i, x, G.lane);
+        const T ax = row_dot<TPR>(A, i, x, G.lane);
         if (G.lane == 0) r[i] = b[i] - ax;
     }
 }
 
-template <int TPR>
-__global__ void __launch_bounds__(kBlock) vc_spmv(int64_t n, CsrOp<double> A, const double* __restrict__ x,
-                                                  double* __restrict__ y, const int* done) {
+template <int TPR, typename T>
+__global__ void __launch_bounds__(kBlock) vc_spmv(int64_t n, CsrOp<T> A, const T* __restrict__ x, T* __restrict__ y,
+                                                  const int* done) {
     if (done && *done) return;
     RowGroups<TPR> G;
     for (int64_t i = G.first; i < n; i += G.stride) {
-        const double ax = row_dot<TPR>(A, i, x, G.lane);
+        const T ax = row_dot<TPR>(A, i, x, G.lane);
         if (G.lane == 0) y[i] = ax;
     }
 }
 
-template <int TPR>
-__global__ void __launch_bounds__(kBlock) vc_prolong(int64_t n, CsrOp<double> P, const double* __restrict__ e,
-                                                     double* __restrict__ x, const int* done) {
+template <int TPR, typename T>
+__global__ void __launch_bounds__(kBlock) vc_prolong(int64_t n, CsrOp<T> P, const T* __restrict__ e,
+                                                     T* __restrict__ x, const int* done) {
     if (done && *done) return;
     RowGroups<TPR> G;
     for (int64_t i = G.first; i < n; i += G.stride) {
-        const double pe = row_dot<TPR>(P, i, e, G.lane);
+        const T pe = row_dot<TPR>(P, i, e, G.lane);
         if (G.lane == 0) x[i] = x[i] + pe;
     }
 }
 
-__global__ void __launch_bounds__(kBlock) vc_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) vc_copy(int64_t n, const T* __restrict__ a, T* __restrict__ b,
                                                   const int* done) {
     if (done && *done) return;
     int64_t lo, hi;
@@ -640,25 +679,26 @@ __global__ void __launch_bounds__(kBlock) vc_copy(int64_t n, const double* __res
 }
 
 // workgroup partial sums of z (the PCG consumer turns them into the mean)
-__global__ void __launch_bounds__(kBlock) k_sum(int64_t n, const double* __restrict__ z, const int* done,
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sum(int64_t n, const T* __restrict__ z, const int* done,
                                                 double* sum_partials) {
     if (done && *done) return;
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc[0] += z[i];
     block_partials<1>(acc, sum_partials);
 }
 
 // x = Minv b for the dense coarse inverse (row per wave)
-__global__ void __launch_bounds__(kBlock) vc_coarse(int64_t n, const double* __restrict__ M,
-                                                    const double* __restrict__ b, double* __restrict__ x,
-                                                    const int* done) {
+template <typename T>
+__global__ void __launch_bounds__(kBlock) vc_coarse(int64_t n, const T* __restrict__ M, const T* __restrict__ b,
+                                                    T* __restrict__ x, const int* done) {
     if (done && *done) return;
     const int lane = threadIdx.x % kWave;
     const int64_t row = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     if (row >= n) return;
-    double s = 0.0;
+    T s = T(0);
     for (int64_t j = lane; j < n; j += kWave) s += M[row * n + j] * b[j];
     s = wave_sum(s);
     if (lane == 0) x[row] = s;
@@ -667,10 +707,10 @@ __global__ void __launch_bounds__(kBlock) vc_coarse(int64_t n, const double* __r
 #define TPR_LAUNCH(tpr, KERNEL, n, st, ...)                                                              \
     do {                                                                                                 \
         switch (tpr) {                                                                                   \
-            case 1: KERNEL<1><<<grid_rows(n, 1), kBlock, 0, st>>>(__VA_ARGS__); break;                    \
-            case 4: KERNEL<4><<<grid_rows(n, 4), kBlock, 0, st>>>(__VA_ARGS__); break;                    \
-            case 16: KERNEL<16><<<grid_rows(n, 16), kBlock, 0, st>>>(__VA_ARGS__); break;                 \
-            default: KERNEL<64><<<grid_rows(n, 64), kBlock, 0, st>>>(__VA_ARGS__); break;                 \
+            case 1: KERNEL<1, T><<<grid_rows(n, 1), kBlock, 0, st>>>(__VA_ARGS__); break;                 \
+            case 4: KERNEL<4, T><<<grid_rows(n, 4), kBlock, 0, st>>>(__VA_ARGS__); break;                 \
+            case 16: KERNEL<16, T><<<grid_rows(n, 16), kBlock, 0, st>>>(__VA_ARGS__); break;              \
+            default: KERNEL<64, T><<<grid_rows(n, 64), kBlock, 0, st>>>(__VA_ARGS__); break;              \
         }                                                                                                \
     } while (0)
 
@@ -680,100 +720,105 @@ inline int sum_grid(int64_t n, int tpr) {
     return g < kRedGrid ? g : kRedGrid;
 }
 
-template <int TPR>
-static void launch_jacobi(int64_t n, CsrOp<double> A, const double* Dinv, const double* b, const double* x,
-                          double* y, double w, const int* done, double* zsum, hipStream_t st) {
+template <int TPR, typename T>
+static void launch_jacobi(int64_t n, CsrOp<T> A, const T* Dinv, const T* b, const T* x, T* y, double w,
+                          const int* done, double* zsum, hipStream_t st) {
     if (zsum)
-        vc_jacobi<TPR, true><<<sum_grid(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, zsum);
+        vc_jacobi<TPR, true, T><<<sum_grid(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, zsum);
     else
-        vc_jacobi<TPR, false><<<grid_rows(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, nullptr);
+        vc_jacobi<TPR, false, T><<<grid_rows(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, nullptr);
 }
 
-static void jacobi(int tpr, int64_t n, CsrOp<double> A, const double* Dinv, const double* b, const double* x,
-                   double* y, double w, const int* done, double* zsum, hipStream_t st) {
+template <typename T>
+static void jacobi(int tpr, int64_t n, CsrOp<T> A, const T* Dinv, const T* b, const T* x, T* y, double w,
+                   const int* done, double* zsum, hipStream_t st) {
     switch (tpr) {
-        case 1: launch_jacobi<1>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
-        case 4: launch_jacobi<4>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
-        case 16: launch_jacobi<16>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
-        default: launch_jacobi<64>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        case 1: launch_jacobi<1, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        case 4: launch_jacobi<4, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        case 16: launch_jacobi<16, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        default: launch_jacobi<64, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
     }
 }
 
 // Enqueue one V-cycle on level k: x_k = M_k^{-1} b_k. When `zsum` is given, the last
 // smoothing sweep also writes the workgroup partial sums of the output (sum_grid of them).
-static void vcycle(const Amg& M, int k, hipStream_t st, const int* done, double* zsum) {
+template <typename T>
+static void vcycle(const Amg<T>& M, int k, hipStream_t st, const int* done, double* zsum) {
     if (k == M.nlev) {
         const int rows_per_block = kBlock / kWave;
-        vc_coarse<<<(int)((M.nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M.nc, M.cinv, M.cb, M.cx,
-                                                                                          done);
+        vc_coarse<T><<<(int)((M.nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M.nc, M.cinv, M.cb,
+                                                                                             M.cx, done);
         return;
     }
-    const Level& L = M.lv[k];
+    const Level<T>& L = M.lv[k];
     const double w = M.omega;
-    double* next_b = (k + 1 == M.nlev) ? M.cb : M.lv[k + 1].b;
-    double* next_x = (k + 1 == M.nlev) ? M.cx : M.lv[k + 1].x;
+    T* next_b = (k + 1 == M.nlev) ? M.cb : M.lv[k + 1].b;
+    T* next_x = (k + 1 == M.nlev) ? M.cx : M.lv[k + 1].x;
     const int64_t next_n = (k + 1 == M.nlev) ? M.nc : M.lv[k + 1].n;
     // pre-smoothing: `sweeps` weighted-Jacobi sweeps from zero
-    double* xa = L.x;
-    double* xb = L.y;
+    T* xa = L.x;
+    T* xb = L.y;
     if (M.sweeps >= 2) {
         TPR_LAUNCH(L.tpr_a, vc_pre2, L.n, st, L.n, L.A, L.Dinv, L.b, xa, w, done);
         for (int s = 2; s < M.sweeps; ++s) {
-            jacobi(L.tpr_a, L.n, L.A, L.Dinv, L.b, xa, xb, w, done, nullptr, st);
-            double* t = xa;
+            jacobi<T>(L.tpr_a, L.n, L.A, L.Dinv, L.b, xa, xb, w, done, nullptr, st);
+            T* t = xa;
             xa = xb;
             xb = t;
         }
     } else {
-        vc_pre1<<<grid_for(L.n), kBlock, 0, st>>>(L.n, L.Dinv, L.b, xa, w, done);
+        vc_pre1<T><<<grid_for(L.n), kBlock, 0, st>>>(L.n, L.Dinv, L.b, xa, w, done);
     }
     TPR_LAUNCH(L.tpr_a, vc_resid, L.n, st, L.n, L.A, L.b, xa, L.r, done);
     TPR_LAUNCH(L.tpr_r, vc_spmv, next_n, st, next_n, L.R, L.r, next_b, done);
     vcycle(M, k + 1, st, done, nullptr);
     TPR_LAUNCH(L.tpr_p, vc_prolong, L.n, st, L.n, L.P, next_x, xa, done);
     // post-smoothing: ping-pong between the level's two buffers, never in place
-    double* cur = xa;
-    double* other = (xa == L.x) ? L.y : L.x;
+    T* cur = xa;
+    T* other = (xa == L.x) ? L.y : L.x;
     for (int s = 0; s < M.sweeps; ++s) {
         const bool last = (s == M.sweeps - 1);
-        jacobi(L.tpr_a, L.n, L.A, L.Dinv, L.b, cur, other, w, done, last ? zsum : nullptr, st);
-        double* t = cur;
+        jacobi<T>(L.tpr_a, L.n, L.A, L.Dinv, L.b, cur, other, w, done, last ? zsum : nullptr, st);
+        T* t = cur;
         cur = other;
         other = t;
     }
-    if (cur != L.x) vc_copy<<<grid_for(L.n), kBlock, 0, st>>>(L.n, cur, L.x, done);
+    if (cur != L.x) vc_copy<T><<<grid_for(L.n), kBlock, 0, st>>>(L.n, cur, L.x, done);
 }
 
 }  // namespace
 
-extern "C" {
 
-void* ftmi_amg_create(double omega, int sweeps) {
-    Amg* M = new Amg();
-    M->omega = omega;
-    M->sweeps = sweeps;
-    return M;
+// ------------------------------------------------------------------ hierarchy set-up (host)
+namespace {
+
+template <typename T>
+Amg<T>* amg_of(void* h, const char* where) {
+    AmgBase* b = (AmgBase*)h;
+    if (!b || b->bits != 8 * (int)sizeof(T)) {
+        set_error_msg(where);
+        return nullptr;
+    }
+    return (Amg<T>*)b;
 }
 
-void ftmi_amg_destroy(void* h) { delete (Amg*)h; }
-
-int ftmi_amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const double* a_val,
-                       const int32_t* p_ptr, const int32_t* p_col, const double* p_val, const int32_t* r_ptr,
-                       const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
-                       double* vy, double* vr) {
-    Amg* M = (Amg*)h;
+template <typename T>
+int amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const T* a_val,
+                  const int32_t* p_ptr, const int32_t* p_col, const T* p_val, const int32_t* r_ptr,
+                  const int32_t* r_col, const T* r_val, const T* dinv, T* vb, T* vx, T* vy, T* vr) {
+    Amg<T>* M = amg_of<T>(h, "ftmi_amg_add_level: handle precision does not match the values");
+    if (!M) return -1;
     if (M->nlev >= 24) {
         set_error_msg("ftmi_amg_add_level: at most 24 levels");
         return -1;
     }
-    Level& L = M->lv[M->nlev++];
+    Level<T>& L = M->lv[M->nlev++];
     L.n = n;
-    L.A = CsrOp<double>{a_ptr, a_col, a_val};
-    L.P = CsrOp<double>{p_ptr, p_col, p_val};
-    L.R = CsrOp<double>{r_ptr, r_col, r_val};
+    L.A = CsrOp<T>{a_ptr, a_col, a_val};
+    L.P = CsrOp<T>{p_ptr, p_col, p_val};
+    L.R = CsrOp<T>{r_ptr, r_col, r_val};
     // row lengths from the CSR pointers (host copies of the last pointer entries)
-    int32_t nnz_a = 0, nnz_p = 0, nnz_r = 0;
-    int64_t nrows_r = 0;
+    int32_t nnz_a = 0, nnz_p = 0;
     if (hipMemcpy(&nnz_a, a_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(&nnz_p, p_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
         set_error_msg("ftmi_amg_add_level: cannot read row pointers");
@@ -784,8 +829,6 @@ int ftmi_amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* 
     L.nnz_p = nnz_p;
     L.tpr_r = 1;  // R = P^T has the next level's rows: set when that size is known
     if (M->nlev >= 2) M->lv[M->nlev - 2].tpr_r = pick_tpr(n, M->lv[M->nlev - 2].nnz_p);
-    (void)nnz_r;
-    (void)nrows_r;
     L.Dinv = dinv;
     L.b = vb;
     L.x = vx;
@@ -794,8 +837,10 @@ int ftmi_amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* 
     return 0;
 }
 
-int ftmi_amg_set_coarse(void* h, int64_t n, const double* inv, double* vb, double* vx) {
-    Amg* M = (Amg*)h;
+template <typename T>
+int amg_set_coarse(void* h, int64_t n, const T* inv, T* vb, T* vx) {
+    Amg<T>* M = amg_of<T>(h, "ftmi_amg_set_coarse: handle precision does not match the values");
+    if (!M) return -1;
     M->nc = n;
     if (M->nlev >= 1) M->lv[M->nlev - 1].tpr_r = pick_tpr(n, M->lv[M->nlev - 1].nnz_p);
     M->cinv = inv;
@@ -804,27 +849,84 @@ int ftmi_amg_set_coarse(void* h, int64_t n, const double* inv, double* vb, doubl
     return 0;
 }
 
-// x = M^{-1} b with one V-cycle (b, x: level-0 vectors, distinct from the level buffers).
-int ftmi_amg_vcycle(void* h, const double* b, double* x, void* stream) {
-    Amg* M = (Amg*)h;
+template <typename T>
+int amg_vcycle(void* h, const T* b, T* x, void* stream) {
+    Amg<T>* M = amg_of<T>(h, "ftmi_amg_vcycle: handle precision does not match the vectors");
+    if (!M) return -1;
     hipStream_t st = (hipStream_t)stream;
     if (M->nlev == 0) {
         const int rows_per_block = kBlock / kWave;
-        vc_coarse<<<(int)((M->nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M->nc, M->cinv, b, x,
-                                                                                           nullptr);
+        vc_coarse<T><<<(int)((M->nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M->nc, M->cinv, b, x,
+                                                                                              nullptr);
         FTMI_LAUNCH_CHECK("ftmi_amg_vcycle");
         return 0;
     }
-    Level& L0 = M->lv[0];
-    double *sb = L0.b, *sx = L0.x;
-    L0.b = (double*)b;
+    Level<T>& L0 = M->lv[0];
+    T *sb = L0.b, *sx = L0.x;
+    L0.b = (T*)b;
     L0.x = x;
-    vcycle(*M, 0, st, nullptr, nullptr);
+    vcycle<T>(*M, 0, st, nullptr, nullptr);
     L0.b = sb;
     L0.x = sx;
     FTMI_LAUNCH_CHECK("ftmi_amg_vcycle");
     return 0;
 }
+
+}  // namespace
+
+extern "C" {
+
+void* ftmi_amg_create(double omega, int sweeps) {
+    Amg<double>* M = new Amg<double>();
+    M->omega = omega;
+    M->sweeps = sweeps;
+    return M;
+}
+
+void* ftmi_amg_create_f32(double omega, int sweeps) {
+    Amg<float>* M = new Amg<float>();
+    M->omega = omega;
+    M->sweeps = sweeps;
+    return M;
+}
+
+void ftmi_amg_destroy(void* h) {
+    AmgBase* b = (AmgBase*)h;
+    if (!b) return;
+    if (b->bits == 32)
+        delete (Amg<float>*)b;
+    else
+        delete (Amg<double>*)b;
+}
+
+int ftmi_amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const double* a_val,
+                       const int32_t* p_ptr, const int32_t* p_col, const double* p_val, const int32_t* r_ptr,
+                       const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
+                       double* vy, double* vr) {
+    return amg_add_level<double>(h, n, a_ptr, a_col, a_val, p_ptr, p_col, p_val, r_ptr, r_col, r_val, dinv, vb, vx,
+                                 vy, vr);
+}
+
+int ftmi_amg_add_level_f32(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const float* a_val,
+                           const int32_t* p_ptr, const int32_t* p_col, const float* p_val, const int32_t* r_ptr,
+                           const int32_t* r_col, const float* r_val, const float* dinv, float* vb, float* vx,
+                           float* vy, float* vr) {
+    return amg_add_level<float>(h, n, a_ptr, a_col, a_val, p_ptr, p_col, p_val, r_ptr, r_col, r_val, dinv, vb, vx,
+                                vy, vr);
+}
+
+int ftmi_amg_set_coarse(void* h, int64_t n, const double* inv, double* vb, double* vx) {
+    return amg_set_coarse<double>(h, n, inv, vb, vx);
+}
+
+int ftmi_amg_set_coarse_f32(void* h, int64_t n, const float* inv, float* vb, float* vx) {
+    return amg_set_coarse<float>(h, n, inv, vb, vx);
+}
+
+// x = M^{-1} b with one V-cycle (b, x: level-0 vectors, distinct from the level buffers).
+int ftmi_amg_vcycle(void* h, const double* b, double* x, void* stream) { return amg_vcycle<double>(h, b, x, stream); }
+
+int ftmi_amg_vcycle_f32(void* h, const float* b, float* x, void* stream) { return amg_vcycle<float>(h, b, x, stream); }
 
 }  // extern "C"
 
@@ -834,11 +936,11 @@ int ftmi_amg_vcycle(void* h, const double* b, double* x, void* stream) {
 // projection of z with r.z. p alternates between two buffers.
 namespace {
 
-template <bool PROJ>
-__global__ void __launch_bounds__(kBlock) pcg_init_a(int64_t n, const double* __restrict__ b, double* part_out) {
+template <bool PROJ, typename T>
+__global__ void __launch_bounds__(kBlock) pcg_init_a(int64_t n, const T* __restrict__ b, double* part_out) {
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[2] = {0.0, 0.0};
+    T acc[2] = {T(0), T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         acc[0] += b[i];
         acc[1] += b[i] * b[i];
@@ -846,26 +948,28 @@ __global__ void __launch_bounds__(kBlock) pcg_init_a(int64_t n, const double* __
     block_partials<2>(acc, part_out);
 }
 
-// bnorm, mean(b); r = b - mean (projection) or b; x = 0; partials r.r
-template <bool PROJ>
-__global__ void __launch_bounds__(kBlock) pcg_init_b(int64_t n, const double* __restrict__ b, double* __restrict__ r,
-                                                     double* __restrict__ x, State* S, const double* part_in,
+// bnorm, mean(b); r = b - mean (projection) or b; x = 0 unless the solve adds to x (then it
+// solves for the correction of the given x, whose residual b is); partials r.r
+template <bool PROJ, typename T, typename XT>
+__global__ void __launch_bounds__(kBlock) pcg_init_b(int64_t n, const T* __restrict__ b, T* __restrict__ r,
+                                                     XT* __restrict__ x, int zero_x, State* S, const double* part_in,
                                                      int np_in, double* part_out) {
     double tot[2];
     consume_sum<2>(part_in, np_in, tot);
-    const double mean = PROJ ? tot[0] / (double)n : 0.0;
+    const double mean_d = PROJ ? tot[0] / (double)n : 0.0;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         const double bn = sqrt(tot[1]);
         S->sc[S_BNORM] = bn != 0.0 ? bn : 1.0;
-        S->sc[S_BMEAN] = mean;
+        S->sc[S_BMEAN] = mean_d;
     }
+    const T mean = (T)mean_d;
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double ri = PROJ ? b[i] - mean : b[i];
+        const T ri = PROJ ? b[i] - mean : b[i];
         r[i] = ri;
-        x[i] = 0.0;
+        if (zero_x) x[i] = XT(0);
         acc[0] += ri * ri;
     }
     block_partials<1>(acc, part_out);
@@ -886,21 +990,22 @@ __global__ void __launch_bounds__(kBlock) fin_pcg_init(const double* partials, i
 }
 
 // z -= mean(z) (mean from the V-cycle's partial sums); partials r.z
-template <bool PROJ>
-__global__ void __launch_bounds__(kBlock) pcg_z(int64_t n, const double* __restrict__ r, double* __restrict__ z,
-                                                const State* S, const double* zsum_in, int np_zsum, double* part_out) {
+template <bool PROJ, typename T>
+__global__ void __launch_bounds__(kBlock) pcg_z(int64_t n, const T* __restrict__ r, T* __restrict__ z, const State* S,
+                                                const double* zsum_in, int np_zsum, double* part_out) {
     if (S->st[I_DONE]) return;
-    double mean = 0.0;
+    double mean_d = 0.0;
     if (PROJ) {
         double tot[1];
         consume_sum<1>(zsum_in, np_zsum, tot);
-        mean = tot[0] / (double)n;
+        mean_d = tot[0] / (double)n;
     }
+    const T mean = (T)mean_d;
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        double zi = z[i];
+        T zi = z[i];
         if (PROJ) {
             zi = zi - mean;
             z[i] = zi;
@@ -912,17 +1017,16 @@ __global__ void __launch_bounds__(kBlock) pcg_z(int64_t n, const double* __restr
 
 // rz_new from the partials; first: p = z, rz = rz_new; else breakdown if rz == 0, beta =
 // rz_new / rz, p = z + beta p_prev; then q = A p and partials p.q
-template <class Op>
-__global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const double* __restrict__ z,
-                                                 const double* __restrict__ p_prev, double* __restrict__ p,
-                                                 double* __restrict__ q, State* S, const double* part_in, int np_in,
-                                                 double* part_out, int first, int k) {
+template <class Op, typename T>
+__global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const T* __restrict__ z, const T* __restrict__ p_prev,
+                                                 T* __restrict__ p, T* __restrict__ q, State* S, const double* part_in,
+                                                 int np_in, double* part_out, int first, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
     consume_sum<1>(part_in, np_in, tot);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int cur = S_RZ + (k & 1), prev = S_RZ + ((k - 1) & 1);
-    double beta = 0.0;
+    double beta_d = 0.0;
     if (!first) {
         const double rz = S->sc[prev];
         if (rz == 0.0) {  // breakdown found at the end of the previous iteration
@@ -933,26 +1037,29 @@ __global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const double* 
             }
             return;
         }
-        beta = tot[0] / rz;
+        beta_d = tot[0] / rz;
     }
     if (lead) S->sc[cur] = tot[0];
+    const T beta = (T)beta_d;
     auto pf = [&](int64_t j) { return first ? z[j] : z[j] + beta * p_prev[j]; };
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double pi = pf(i);
+        const T pi = pf(i);
         p[i] = pi;
-        const double qi = A.row_f(i, pf);
+        const T qi = A.row_f(i, pf);
         q[i] = qi;
         acc[0] += pi * qi;
     }
     block_partials<1>(acc, part_out);
 }
 
-// alpha = rz / (p.q) (breakdown if p.q <= 0); x += alpha p; r -= alpha q; partials r.r
-__global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const double* __restrict__ p, const double* __restrict__ q,
-                                                 double* __restrict__ x, double* __restrict__ r, State* S,
+// alpha = rz / (p.q) (breakdown if p.q <= 0); x += alpha p (in the iterate's precision XT);
+// r -= alpha q; partials r.r
+template <typename T, typename XT>
+__global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const T* __restrict__ p, const T* __restrict__ q,
+                                                 XT* __restrict__ x, T* __restrict__ r, State* S,
                                                  const double* part_in, int np_in, double* part_out, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
@@ -966,13 +1073,15 @@ __global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const double* __rest
         }
         return;
     }
-    const double a = S->sc[S_RZ + (k & 1)] / tot[0];
+    const double a_d = S->sc[S_RZ + (k & 1)] / tot[0];
+    const T a = (T)a_d;
+    const XT ax = (XT)a_d;
     int64_t lo, hi;
     block_range(n, lo, hi);
-    double acc[1] = {0.0};
+    T acc[1] = {T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        x[i] += a * p[i];
-        const double ri = r[i] - a * q[i];
+        x[i] += ax * (XT)p[i];
+        const T ri = r[i] - a * q[i];
         r[i] = ri;
         acc[0] += ri * ri;
     }
@@ -994,90 +1103,98 @@ __global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int
     }
 }
 
+template <typename T>
 struct PcgWork {
-    double *r, *z, *p[2], *q;
+    T *r, *z, *p[2], *q;
     State* S;
     unsigned* counter;
     double* part[3];
 };
 
-static PcgWork carve_pcg(int64_t n, void* work) {
+template <typename T>
+static int64_t pcg_bytes(int64_t n) {
+    return align256(5 * n * (int64_t)sizeof(T)) + 512 + 3 * kPartBytes;
+}
+
+template <typename T>
+static PcgWork<T> carve_pcg(int64_t n, void* work) {
     char* w = (char*)work;
-    PcgWork W;
-    double* d = (double*)w;
+    PcgWork<T> W;
+    T* d = (T*)w;
     W.r = d;
     W.z = d + n;
     W.p[0] = d + 2 * n;
     W.p[1] = d + 3 * n;
     W.q = d + 4 * n;
-    char* tail = w + ((5 * n * sizeof(double) + 255) / 256) * 256;
+    char* tail = w + align256(5 * n * (int64_t)sizeof(T));
     W.S = (State*)tail;
     W.counter = (unsigned*)(tail + 256);
     for (int j = 0; j < 3; ++j) W.part[j] = (double*)(tail + 512 + j * kPartBytes);
     return W;
 }
 
-template <class Op>
-int pcg_run(int64_t n, Op A, const double* b, double* x, double tol, int64_t maxiter, void* amg, int project,
-            void* work, int64_t* iters, double* res, hipStream_t st) {
+// x in the iterate precision XT; zero_x = false adds the solve's result to the given x
+template <class Op, typename T, typename XT = T>
+int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg<T>* M, int project, void* work,
+            int64_t* iters, double* res, hipStream_t st, bool zero_x = true) {
     Pinned* pn = pinned();
     if (!pn) {
         set_error_msg("pcg: pinned host memory unavailable");
         return -1;
     }
-    Amg* M = (Amg*)amg;
-    PcgWork W = carve_pcg(n, work);
+    PcgWork<T> W = carve_pcg<T>(n, work);
     const int g = grid_red(n);
     const int* done = &W.S->st[I_DONE];
     double *P0 = W.part[0], *P1 = W.part[1], *P2 = W.part[2];
     k_state_reset<<<1, 64, 0, st>>>(W.S, W.counter);
+    const int zx = zero_x ? 1 : 0;
     if (project) {
-        pcg_init_a<true><<<g, kBlock, 0, st>>>(n, b, P0);
-        pcg_init_b<true><<<g, kBlock, 0, st>>>(n, b, W.r, x, W.S, P0, g, P1);
+        pcg_init_a<true, T><<<g, kBlock, 0, st>>>(n, b, P0);
+        pcg_init_b<true, T, XT><<<g, kBlock, 0, st>>>(n, b, W.r, x, zx, W.S, P0, g, P1);
     } else {
-        pcg_init_a<false><<<g, kBlock, 0, st>>>(n, b, P0);
-        pcg_init_b<false><<<g, kBlock, 0, st>>>(n, b, W.r, x, W.S, P0, g, P1);
+        pcg_init_a<false, T><<<g, kBlock, 0, st>>>(n, b, P0);
+        pcg_init_b<false, T, XT><<<g, kBlock, 0, st>>>(n, b, W.r, x, zx, W.S, P0, g, P1);
     }
     fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol);
     // z = M r, with the partial sums of z in P2 when projecting; returns their count
     auto precond = [&]() -> int {
         if (M && M->nlev > 0) {
-            Level& L0 = M->lv[0];
-            double *sb = L0.b, *sx = L0.x;
+            Level<T>& L0 = M->lv[0];
+            T *sb = L0.b, *sx = L0.x;
             L0.b = W.r;
             L0.x = W.z;
-            vcycle(*M, 0, st, done, project ? P2 : nullptr);
+            vcycle<T>(*M, 0, st, done, project ? P2 : nullptr);
             L0.b = sb;
             L0.x = sx;
             return sum_grid(n, L0.tpr_a);
         }
         if (M) {
             const int rpb = kBlock / kWave;
-            vc_coarse<<<(int)((M->nc + rpb - 1) / rpb), kBlock, 0, st>>>(M->nc, M->cinv, W.r, W.z, done);
+            vc_coarse<T><<<(int)((M->nc + rpb - 1) / rpb), kBlock, 0, st>>>(M->nc, M->cinv, W.r, W.z, done);
         } else {
-            vc_copy<<<grid_for(n), kBlock, 0, st>>>(n, W.r, W.z, done);
+            vc_copy<T><<<grid_for(n), kBlock, 0, st>>>(n, W.r, W.z, done);
         }
-        if (project) k_sum<<<g, kBlock, 0, st>>>(n, W.z, done, P2);
+        if (project) k_sum<T><<<g, kBlock, 0, st>>>(n, W.z, done, P2);
         return g;
     };
     int nz = precond();
     if (project)
-        pcg_z<true><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+        pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
     else
-        pcg_z<false><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+        pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
     FTMI_LAUNCH_CHECK("pcg init");
     if (post_state(pn, 0, W.S, st)) return -1;
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-        pcg_ap<Op><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
-        pcg_xr<<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
+        pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
+        pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
         fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k);
         nz = precond();
         if (project)
-            pcg_z<true><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+            pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         else
-            pcg_z<false><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+            pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         FTMI_LAUNCH_CHECK("pcg iteration");
         if (post_state(pn, last ^ 1, W.S, st)) return -1;
         if (wait_state(pn, last)) return -1;
@@ -1098,26 +1215,208 @@ int pcg_run(int64_t n, Op A, const double* b, double* x, double tol, int64_t max
     return 0;
 }
 
+// ------------------------------------------------------------------ mixed precision
+// FP32 right-hand side of a correction solve: rb = b - A x with FP32 matrix and arithmetic
+// and the FP64 iterate rounded on load
+__global__ void __launch_bounds__(kBlock) k_resid_f32(int64_t n, LduDiffOp<float> A, const float* __restrict__ b,
+                                                      const double* __restrict__ x, float* __restrict__ rb) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    auto xf = [&](int64_t j) { return (float)x[j]; };
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) rb[i] = b[i] - A.row_f(i, xf);
+}
+
+// x += d (FP32 correction added to the FP64 iterate)
+__global__ void __launch_bounds__(kBlock) k_add_f32(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) x[i] += (double)d[i];
+}
+
+// FP64 residual of the pressure system: r = b - A x; partials sum r, sum r^2, sum b^2
+__global__ void __launch_bounds__(kBlock) k_resid_f64(int64_t n, LduOp<double> A, const double* __restrict__ b,
+                                                      const double* __restrict__ x, double* __restrict__ r,
+                                                      double* part_out) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const double ri = b[i] - A.row(i, x);
+        r[i] = ri;
+        acc[0] += ri;
+        acc[1] += ri * ri;
+        acc[2] += b[i] * b[i];
+    }
+    block_partials<3>(acc, part_out);
+}
+
+// r -= mean (the projected residual), rounded to FP32 into r32; partials of the projected r^2
+__global__ void __launch_bounds__(kBlock) k_project_round(int64_t n, double* __restrict__ r, const double* part_in,
+                                                          int np_in, int project, float* __restrict__ r32,
+                                                          double* part_out) {
+    double tot[3];
+    consume_sum<3>(part_in, np_in, tot);
+    const double mean = project ? tot[0] / (double)n : 0.0;
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    double acc[1] = {0.0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const double ri = r[i] - mean;
+        r[i] = ri;
+        r32[i] = (float)ri;
+        acc[0] += ri * ri;
+    }
+    block_partials<1>(acc, part_out);
+}
+
+// out[0] = ||b||^2, out[1] = ||projected r||^2
+__global__ void __launch_bounds__(kBlock) fin_mixed_norms(const double* p3, const double* p1, int nparts, State* S) {
+    double t3[3], t1[1];
+    const bool lead3 = finalize_sum<3>(p3, nparts, t3);
+    const bool lead1 = finalize_sum<1>(p1, nparts, t1);
+    if (lead3 && lead1) {
+        S->sc[S_BABS] = sqrt(t3[2]);
+        S->sc[S_RNORM] = sqrt(t1[0]);
+    }
+}
+
+// ex = (float)(d + sum_k off_k): the row sums of the FP64 operator, rounded once
+__global__ void __launch_bounds__(kBlock) k_rowsum_f32(int64_t n, LduOp<double> A, float* __restrict__ ex) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        double s = A.diag[i];
+        for (int32_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k) s += A.off[k];
+        ex[i] = (float)s;
+    }
+}
+
+static int64_t pcg_mixed_bytes(int64_t n) {
+    return align256(n * 8) + align256(2 * n * 4) + align256(sizeof(State)) + 2 * kPartBytes + pcg_bytes<float>(n);
+}
+
 }  // namespace
 
 extern "C" {
 
 int64_t ftmi_pcg_work_bytes(int64_t n) {
-    return ((5 * n * (int64_t)sizeof(double) + 255) / 256) * 256 + 512 + 3 * kPartBytes;
+    const int64_t a = pcg_bytes<double>(n), b = pcg_mixed_bytes(n);
+    return a > b ? a : b;
 }
 
 int ftmi_pcg_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
                  const double* b, double* x, double tol, int64_t maxiter, void* amg, int project, void* work,
                  int64_t* iters, double* res, void* stream) {
+    Amg<double>* M = nullptr;
+    if (amg && !(M = amg_of<double>(amg, "ftmi_pcg_ldu: the preconditioner is not an FP64 hierarchy"))) return -1;
     LduOp<double> A{diag, ptr, nbr, off};
-    return pcg_run(n, A, b, x, tol, maxiter, amg, project, work, iters, res, (hipStream_t)stream);
+    return pcg_run<LduOp<double>, double>(n, A, b, x, tol, maxiter, M, project, work, iters, res, (hipStream_t)stream);
 }
 
 int ftmi_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
                  double* x, double tol, int64_t maxiter, void* amg, int project, void* work, int64_t* iters,
                  double* res, void* stream) {
+    Amg<double>* M = nullptr;
+    if (amg && !(M = amg_of<double>(amg, "ftmi_pcg_csr: the preconditioner is not an FP64 hierarchy"))) return -1;
     CsrOp<double> A{indptr, indices, data};
-    return pcg_run(n, A, b, x, tol, maxiter, amg, project, work, iters, res, (hipStream_t)stream);
+    return pcg_run<CsrOp<double>, double>(n, A, b, x, tol, maxiter, M, project, work, iters, res, (hipStream_t)stream);
+}
+
+int ftmi_pcg_ldu_f32(int64_t n, const float* diag, const int32_t* ptr, const int32_t* nbr, const float* off,
+                     const float* b, float* x, double tol, int64_t maxiter, void* amg, int project, void* work,
+                     int64_t* iters, double* res, void* stream) {
+    Amg<float>* M = nullptr;
+    if (amg && !(M = amg_of<float>(amg, "ftmi_pcg_ldu_f32: the preconditioner is not an FP32 hierarchy"))) return -1;
+    LduOp<float> A{diag, ptr, nbr, off};
+    return pcg_run<LduOp<float>, float>(n, A, b, x, tol, maxiter, M, project, work, iters, res, (hipStream_t)stream);
+}
+
+int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, const int32_t* nbr, const float* off,
+                            const float* b, const float* inv_diag, double* x, double tol, int64_t maxiter,
+                            int relative_to_initial, void* work, int64_t* iters, double* res, void* stream) {
+    if (n <= 0) {
+        *iters = 0;
+        *res = 0.0;
+        return 1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    LduDiffOp<float> A{ex, ptr, nbr, off};
+    float* rb = (float*)work;
+    float* d = rb + n;
+    void* inner = (char*)work + align256(2 * n * (int64_t)sizeof(float));
+    k_resid_f32<<<grid_for(n), kBlock, 0, st>>>(n, A, b, x, rb);
+    FTMI_LAUNCH_CHECK("ftmi_bicgstab_ldu_mixed residual");
+    const int conv = bicgstab_run<LduDiffOp<float>, float>(n, A, rb, inv_diag, d, tol, maxiter, relative_to_initial,
+                                                           inner, iters, res, st, true);
+    if (conv < 0) return conv;
+    k_add_f32<<<grid_for(n), kBlock, 0, st>>>(n, d, x);
+    FTMI_LAUNCH_CHECK("ftmi_bicgstab_ldu_mixed update");
+    return conv;
+}
+
+int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
+                       const float* off32, const double* b, double* x, double tol, int64_t maxiter, void* amg32,
+                       int project, int max_refine, void* work, int64_t* iters, double* res, int64_t* rounds,
+                       void* stream) {
+    Amg<float>* M = nullptr;
+    if (amg32 && !(M = amg_of<float>(amg32, "ftmi_pcg_ldu_mixed: the preconditioner is not an FP32 hierarchy")))
+        return -1;
+    Pinned* pn = pinned();
+    if (!pn) {
+        set_error_msg("ftmi_pcg_ldu_mixed: pinned host memory unavailable");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    char* w = (char*)work;
+    double* r64 = (double*)w;
+    w += align256(n * 8);
+    float* r32 = (float*)w;
+    float* ex32 = r32 + n;
+    w += align256(2 * n * 4);
+    State* S = (State*)w;
+    w += align256(sizeof(State));
+    double* pa = (double*)w;
+    double* pb = (double*)(w + kPartBytes);
+    void* inner = w + 2 * kPartBytes;
+    LduOp<double> A64{diag, ptr, nbr, off};
+    LduDiffOp<float> A32{ex32, ptr, nbr, off32};
+    k_rowsum_f32<<<grid_for(n), kBlock, 0, st>>>(n, A64, ex32);
+    const int g = grid_red(n);
+    int64_t total = 0;
+    double bnorm = 1.0, rel = 0.0;
+    int conv = 0;
+    *rounds = 0;
+    for (int round = 0; round <= max_refine; ++round) {
+        // the FP64 residual of the current iterate, projected, and its norm (one host read)
+        k_resid_f64<<<g, kBlock, 0, st>>>(n, A64, b, x, r64, pa);
+        k_project_round<<<g, kBlock, 0, st>>>(n, r64, pa, g, project, r32, pb);
+        fin_mixed_norms<<<1, kBlock, 0, st>>>(pa, pb, g, S);
+        FTMI_LAUNCH_CHECK("ftmi_pcg_ldu_mixed residual");
+        if (post_state(pn, 0, S, st) || wait_state(pn, 0)) return -1;
+        const double babs = pn->h[0].sc[S_BABS], rnorm = pn->h[0].sc[S_RNORM];
+        if (round == 0) bnorm = babs != 0.0 ? babs : 1.0;
+        rel = rnorm / bnorm;
+        if (rel < tol) {
+            conv = 1;
+            break;
+        }
+        if (round == max_refine || total >= maxiter) break;
+        // FP32 PCG on A d = r to the absolute level tol * ||b||; its updates x += alpha p go
+        // straight into the FP64 iterate (an FP32 iterate's rounding, rough from cell to cell,
+        // is not cancelled by the Laplacian and would cap the FP64 residual at fine meshes)
+        int64_t it = 0;
+        double r_in = 0.0;
+        const double tol_in = tol * bnorm / rnorm;
+        const int c = pcg_run<LduDiffOp<float>, float, double>(n, A32, r32, x, tol_in, maxiter - total, M, project,
+                                                               inner, &it, &r_in, st, false);
+        if (c < 0) return c;
+        total += it;
+        ++*rounds;
+        if (it == 0 && c == 0) break;  // breakdown at the start: nothing more to gain
+    }
+    *iters = total;
+    *res = rel;
+    return conv;
 }
 
 }  // extern "C"

@@ -51,4 +51,31 @@ struct LduOp {
     }
 };
 
+// The same operator in difference form: y_i = ex_i x_i + sum_k off_k (x_nbr(k) - x_i) with
+// ex_i = d_i + sum_k off_k (the row sum). For a nearly singular Laplacian (row sums ~ 0) the
+// rounding of FP32 coefficients then multiplies differences of x, so the error of A x scales
+// with |A x| instead of |A| |x|.
+template <typename T>
+struct LduDiffOp {
+    const T* ex;
+    const int32_t* ptr;
+    const int32_t* nbr;
+    const T* off;
+    __device__ inline T row(int64_t i, const T* __restrict__ x) const {
+        const T xi = x[i];
+        T s = ex[i] * xi;
+        const int32_t e = ptr[i + 1];
+        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * (x[nbr[k]] - xi);
+        return s;
+    }
+    template <class F>
+    __device__ inline T row_f(int64_t i, F f) const {
+        const T xi = f(i);
+        T s = ex[i] * xi;
+        const int32_t e = ptr[i + 1];
+        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * (f(nbr[k]) - xi);
+        return s;
+    }
+};
+
 }  // namespace ftmi

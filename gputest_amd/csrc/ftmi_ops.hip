@@ -33,6 +33,12 @@ int ftmi_device_count(void) {
 
 namespace {
 
+__global__ void __launch_bounds__(kBlock) k_cast_f32(int64_t n, const double* __restrict__ x, float* __restrict__ y) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) y[i] = (float)x[i];
+}
+
 // y[i, :] = sum_j A[i, j] X[j, :]   (k right-hand sides, CSR order as ft_csr_spmm)
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_csr_spmm(int64_t n, const int32_t* __restrict__ ptr,
@@ -191,6 +197,12 @@ int ftmi_csr_spmm(int64_t nrows, int64_t k, const int32_t* indptr, const int32_t
     else
         k_csr_spmm<0><<<g, kBlock, 0, s>>>(nrows, indptr, indices, data, X, Y, (int)k);
     FTMI_LAUNCH_CHECK("ftmi_csr_spmm");
+    return 0;
+}
+
+int ftmi_cast_f64_f32(int64_t n, const double* x, float* y, void* stream) {
+    if (n > 0) k_cast_f32<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, x, y);
+    FTMI_LAUNCH_CHECK("ftmi_cast_f64_f32");
     return 0;
 }
 

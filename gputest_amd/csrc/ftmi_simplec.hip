@@ -20,9 +20,11 @@ enum { WALL = 0, INLET = 1, OUTLET = 2, PRESSURE = 3, SYMMETRY = 4 };
 enum { T_ADIABATIC = 0, T_FIXED = 1, T_FLUX = 2 };
 
 inline double dmax(double a, double b) { return a > b ? a : b; }
-__device__ inline double fmax0(double a) { return a > 0.0 ? a : 0.0; }
+template <typename R>
+__device__ inline R fmax0(R a) { return a > R(0) ? a : R(0); }
 
-__device__ inline double clipd(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+template <typename R>
+__device__ inline R clipd(R x, R lo, R hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 struct MeshV {  // device view of ftmi_mesh
     int64_t nc, nf, nb;
@@ -288,32 +290,46 @@ struct AsmArgs {
     int second_order;
     int simplec;        // write dP
     double cp;          // energy: boundary heat flux source q = flux * value * |Sb| / cp
-    double* diag;
-    double* diag_r;
-    double* off;
-    double* b[3];
     double* dP;
     double* resid;  // resid[2*q] = sum |rhs - A phi|, resid[2*q+1] = sum |diag phi|
-    double* inv_r;
     double* x0[3];
 };
 
-template <int DIM, int NF>
-__global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A, RedBuf rb) {
+// Arithmetic and outputs in R: double for the FP64 solver, float for the mixed-precision
+// transport systems (inputs stay FP64 and are rounded on load; dP stays FP64). In FP32 the
+// operator is also written in difference form: ex = diag_r + sum_k off_k, accumulated from
+// the face fluxes (each face adds +-F, the diffusion parts cancelling exactly) plus the
+// boundary and relaxation terms, and the residual sums use A phi = ex phi_P + sum off (phi_N
+// - phi_P); FP32 rounding then scales with the residual, not with |A| |phi|.
+template <typename R>
+struct AsmOut {
+    R* diag;
+    R* diag_r;
+    R* off;
+    R* b[3];
+    R* inv_r;
+    R* ex;  // FP32 only (difference form)
+};
+
+template <int DIM, int NF, typename R>
+__global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A, AsmOut<R> O, RedBuf rb) {
     int64_t lo, hi;
     block_range(m.nc, lo, hi);
-    double acc[2 * NF];
+    R acc[2 * NF];
 #pragma unroll
-    for (int j = 0; j < 2 * NF; ++j) acc[j] = 0.0;
+    for (int j = 0; j < 2 * NF; ++j) acc[j] = R(0);
+    const R gamma = (R)A.gamma;
     for (int64_t c = lo + threadIdx.x; c < hi; c += kBlock) {
-        double dg_o = 0.0, dg_n = 0.0, dg_b = 0.0;   // diagonal: owner, neighbour, boundary sums
-        double nb_o = 0.0, nb_n = 0.0;               // sum of -a_nb (SIMPLEC)
+        constexpr bool kDiff = sizeof(R) == 4;
+        R dg_o = R(0), dg_n = R(0), dg_b = R(0);   // diagonal: owner, neighbour, boundary sums
+        R nb_o = R(0), nb_n = R(0);                // sum of -a_nb (SIMPLEC)
+        R exs = R(0);                              // kDiff: sum of +-F over the faces
         // rt, rd: Cdiff products (owner +, neighbour -, interleaved in face order)
-        double rb_[NF], rt[NF], rd[NF], ax[NF], pc[NF];
+        R rb_[NF], rt[NF], rd[NF], ax[NF], pc[NF];
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
-            rb_[q] = rt[q] = rd[q] = ax[q] = 0.0;
-            pc[q] = A.phi[q][c];
+            rb_[q] = rt[q] = rd[q] = ax[q] = R(0);
+            pc[q] = (R)A.phi[q][c];
         }
         const int32_t e = m.cptr[c + 1];
         for (int32_t k = m.cptr[c]; k < e; ++k) {
@@ -321,12 +337,13 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             const int64_t f = code >> 1;
             const bool isN = code & 1;
             const int64_t o = m.cnbr[k];
-            const double F = A.F[f];
-            const double D = A.gamma * m.alpha[f];
-            const double aPN = -(D + fmax0(-F));
-            const double aNP = -(D + fmax0(F));
-            const double a_here = isN ? aNP : aPN;
-            A.off[k] = a_here;
+            const R F = (R)A.F[f];
+            const R D = gamma * (R)m.alpha[f];
+            const R aPN = -(D + fmax0(-F));
+            const R aNP = -(D + fmax0(F));
+            const R a_here = isN ? aNP : aPN;
+            O.off[k] = a_here;
+            if (kDiff) exs += isN ? -F : F;
             if (isN) {
                 dg_n += D + fmax0(-F);
                 nb_n += -aNP;
@@ -335,30 +352,31 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
                 nb_o += -aPN;
             }
             const int64_t P = isN ? o : c, N = isN ? c : o;
-            const double wf = m.w[f], wn = 1.0 - wf;
+            const R wf = (R)m.w[f], wn = R(1) - wf;
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
                 const double* g = A.grad[q];
-                const double po = A.phi[q][o];
-                ax[q] += a_here * po;
+                const R po = (R)A.phi[q][o];
+                ax[q] += kDiff ? a_here * (po - pc[q]) : a_here * po;
                 // non-orthogonal diffusion gamma (grad phi)_f . T
-                double gt = (wf * g[P * DIM] + wn * g[N * DIM]) * m.Tv[f * DIM];
+                R gt = (wf * (R)g[P * DIM] + wn * (R)g[N * DIM]) * (R)m.Tv[f * DIM];
 #pragma unroll
-                for (int a = 1; a < DIM; ++a) gt += (wf * g[P * DIM + a] + wn * g[N * DIM + a]) * m.Tv[f * DIM + a];
-                const double tn = A.gamma * gt;
-                double dc = 0.0;
+                for (int a = 1; a < DIM; ++a)
+                    gt += (wf * (R)g[P * DIM + a] + wn * (R)g[N * DIM + a]) * (R)m.Tv[f * DIM + a];
+                const R tn = gamma * gt;
+                R dc = R(0);
                 if (A.second_order) {
-                    const double phP = isN ? po : pc[q], phN = isN ? pc[q] : po;
-                    double sP = g[P * DIM] * m.dPf[f * DIM], sN = g[N * DIM] * m.dNf[f * DIM];
+                    const R phP = isN ? po : pc[q], phN = isN ? pc[q] : po;
+                    R sP = (R)g[P * DIM] * (R)m.dPf[f * DIM], sN = (R)g[N * DIM] * (R)m.dNf[f * DIM];
 #pragma unroll
                     for (int a = 1; a < DIM; ++a) {
-                        sP += g[P * DIM + a] * m.dPf[f * DIM + a];
-                        sN += g[N * DIM + a] * m.dNf[f * DIM + a];
+                        sP += (R)g[P * DIM + a] * (R)m.dPf[f * DIM + a];
+                        sN += (R)g[N * DIM + a] * (R)m.dNf[f * DIM + a];
                     }
-                    const bool up = F >= 0.0;
-                    const double hoP = phP + sP, hoN = phN + sN;
-                    const double lo_ = phP < phN ? phP : phN, hi_ = phP > phN ? phP : phN;
-                    const double ho = clipd(up ? hoP : hoN, lo_, hi_);
+                    const bool up = F >= R(0);
+                    const R hoP = phP + sP, hoN = phN + sN;
+                    const R lo_ = phP < phN ? phP : phN, hi_ = phP > phN ? phP : phN;
+                    const R ho = clipd(up ? hoP : hoN, lo_, hi_);
                     dc = F * (ho - (up ? phP : phN));
                 }
                 rt[q] += isN ? -tn : tn;
@@ -366,54 +384,58 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             }
         }
         const int32_t be = m.bptr[c + 1];
-        double qin = 0.0;
+        R qin = R(0);
         for (int32_t j = m.bptr[c]; j < be; ++j) {
             const int64_t f = m.bface[j];
             const int kd = bc.kind[f];
-            const double Fb = A.Fb[f];
-            double dir;
+            const R Fb = (R)A.Fb[f];
+            R dir;
             if (A.dir_mode == 0)
-                dir = m_dirvel(kd);
+                dir = (R)m_dirvel(kd);
             else
-                dir = bc.tkind[f] == T_FIXED ? 1.0 : 0.0;
-            const double Db = A.gamma * m.alpha_b[f] * dir;
+                dir = bc.tkind[f] == T_FIXED ? R(1) : R(0);
+            const R Db = gamma * (R)m.alpha_b[f] * dir;
             dg_b += Db + fmax0(Fb);
-            const double cb = Db + fmax0(-Fb);
+            const R cb = Db + fmax0(-Fb);
 #pragma unroll
-            for (int q = 0; q < NF; ++q) rb_[q] += cb * A.phib[q][f];
-            if (A.dir_mode == 1) qin += (bc.tkind[f] == T_FLUX ? 1.0 : 0.0) * bc.tval[f] * m.Sbm[f] / A.cp;
+            for (int q = 0; q < NF; ++q) rb_[q] += cb * (R)A.phib[q][f];
+            if (A.dir_mode == 1)
+                qin += (bc.tkind[f] == T_FLUX ? R(1) : R(0)) * (R)bc.tval[f] * (R)m.Sbm[f] / (R)A.cp;
         }
-        const double diag = (dg_o + dg_n) + dg_b;
-        const double diag_r = diag / A.relax;
-        A.diag[c] = diag;
-        A.diag_r[c] = diag_r;
-        if (A.inv_r) A.inv_r[c] = 1.0 / diag_r;
-        const double vol = m.vol[c];
-        const double crel = (1.0 - A.relax) / A.relax;
+        const R relax = (R)A.relax;
+        const R diag = (dg_o + dg_n) + dg_b;
+        const R diag_r = diag / relax;
+        O.diag[c] = diag;
+        O.diag_r[c] = diag_r;
+        if (O.inv_r) O.inv_r[c] = R(1) / diag_r;
+        const R vol = (R)m.vol[c];
+        const R crel = (R(1) - relax) / relax;
+        const R ex_u = exs + dg_b;  // kDiff: diag + sum_k off_k of the unrelaxed operator
+        if (kDiff) O.ex[c] = ex_u + crel * diag;
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             // rhs = Cb@(bnd) + Cdiff@(nonorth) - Cdiff@(deferred) + extra
-            double rhs = (rb_[q] + rt[q]) - rd[q];
+            R rhs = (rb_[q] + rt[q]) - rd[q];
             if (A.dir_mode == 0) {
-                double src = -A.gp[c * DIM + q] * vol;
-                if (A.has_drive) src = src + vol * A.drive[q];
+                R src = -(R)A.gp[c * DIM + q] * vol;
+                if (A.has_drive) src = src + vol * (R)A.drive[q];
                 rhs = rhs + src;
             } else {
                 rhs = rhs + qin;
             }
-            const double Aphi = diag * pc[q] + ax[q];
+            const R Aphi = (kDiff ? ex_u : diag) * pc[q] + ax[q];
             acc[2 * q] += fabs(rhs - Aphi);
             acc[2 * q + 1] += fabs(diag * pc[q]);
-            A.b[q][c] = rhs + crel * diag * pc[q];
-            if (A.x0[q]) A.x0[q][c] = pc[q];
+            O.b[q][c] = rhs + crel * diag * pc[q];
+            if (A.x0[q]) A.x0[q][c] = A.phi[q][c];
         }
         if (A.simplec >= 0 && A.dP) {
-            double denom = diag_r;
+            R denom = diag_r;
             if (A.simplec == 1) {
                 denom = diag_r - (nb_o + nb_n);
-                denom = denom > 1e-12 * diag_r ? denom : diag_r;
+                denom = denom > R(1e-12) * diag_r ? denom : diag_r;
             }
-            A.dP[c] = vol / denom;
+            A.dP[c] = (double)(vol / denom);
         }
     }
     block_partials<2 * NF>(acc, rb.partials);
@@ -685,6 +707,30 @@ int ftmi_gg_gradient_fused(const ftmi_mesh* mesh, const double* phi, const doubl
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+template <typename R>
+void launch_assemble(int dim, bool energy, int g, hipStream_t s, const MeshV& m, const BcV& b, const AsmArgs& A,
+                     const AsmOut<R>& O, RedBuf rb) {
+    if (energy) {
+        if (dim == 2)
+            k_assemble<2, 1, R><<<g, kBlock, 0, s>>>(m, b, A, O, rb);
+        else
+            k_assemble<3, 1, R><<<g, kBlock, 0, s>>>(m, b, A, O, rb);
+        fin_store<2><<<1, kBlock, 0, s>>>(rb.partials, g, A.resid, 1.0);
+    } else if (dim == 2) {
+        k_assemble<2, 2, R><<<g, kBlock, 0, s>>>(m, b, A, O, rb);
+        fin_store<4><<<1, kBlock, 0, s>>>(rb.partials, g, A.resid, 1.0);
+    } else {
+        k_assemble<3, 3, R><<<g, kBlock, 0, s>>>(m, b, A, O, rb);
+        fin_store<6><<<1, kBlock, 0, s>>>(rb.partials, g, A.resid, 1.0);
+    }
+}
+}  // namespace
+
+extern "C" {
+
 int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi_assemble_args* a,
                             const ftmi_red* rbuf, void* stream) {
     MeshV m = view(mesh);
@@ -698,7 +744,6 @@ int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi
         A.phi[q] = a->phi[q];
         A.grad[q] = a->grad[q];
         A.phib[q] = a->phib[q];
-        A.b[q] = a->b[q];
     }
     A.gp = a->gp;
     for (int q = 0; q < 3; ++q) A.drive[q] = a->drive[q];
@@ -707,35 +752,26 @@ int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi
     A.second_order = a->second_order;
     A.simplec = a->simplec;
     A.cp = a->cp;
-    A.diag = a->diag;
-    A.diag_r = a->diag_r;
-    A.off = a->off;
     A.dP = a->dP;
     A.resid = a->resid;
-    A.inv_r = a->inv_r;
     for (int q = 0; q < 3; ++q) A.x0[q] = q < nfld ? a->x0[q] : nullptr;
     hipStream_t s = (hipStream_t)stream;
     const int g = grid_for(m.nc);
     RedBuf rb = red(rbuf);
-    if (a->energy) {
-        if (nfld != 1) {
-            set_error_msg("energy assembly takes one field");
+    if (a->energy ? nfld != 1 : nfld != mesh->dim) {
+        set_error_msg(a->energy ? "energy assembly takes one field" : "momentum assembly takes dim fields");
+        return -1;
+    }
+    if (a->out_fp32) {
+        if (!a->ex32) {
+            set_error_msg("ftmi_transport_assemble: out_fp32 needs ex32");
             return -1;
         }
-        DISPATCH_DIM(mesh->dim, (k_assemble<D, 1><<<g, kBlock, 0, s>>>(m, bview(bc), A, rb)));
-        fin_store<2><<<1, kBlock, 0, s>>>(rb.partials, g, A.resid, 1.0);
+        AsmOut<float> O{a->diag32, a->diag_r32, a->off32, {a->b32[0], a->b32[1], a->b32[2]}, a->inv_r32, a->ex32};
+        launch_assemble<float>(mesh->dim, a->energy != 0, g, s, m, bview(bc), A, O, rb);
     } else {
-        if (nfld != mesh->dim) {
-            set_error_msg("momentum assembly takes dim fields");
-            return -1;
-        }
-        if (mesh->dim == 2) {
-            k_assemble<2, 2><<<g, kBlock, 0, s>>>(m, bview(bc), A, rb);
-            fin_store<4><<<1, kBlock, 0, s>>>(rb.partials, g, A.resid, 1.0);
-        } else {
-            k_assemble<3, 3><<<g, kBlock, 0, s>>>(m, bview(bc), A, rb);
-            fin_store<6><<<1, kBlock, 0, s>>>(rb.partials, g, A.resid, 1.0);
-        }
+        AsmOut<double> O{a->diag, a->diag_r, a->off, {a->b[0], a->b[1], a->b[2]}, a->inv_r, nullptr};
+        launch_assemble<double>(mesh->dim, a->energy != 0, g, s, m, bview(bc), A, O, rb);
     }
     FTMI_LAUNCH_CHECK("ftmi_transport_assemble");
     return 0;

@@ -176,6 +176,7 @@ class FVMesh:
         self.indptr_h, self.indices_h, self.slot_diag_h, self.he_slot_h = indptr, indices, slot_diag, he_slot
         self.slot_PN_h, self.slot_NP_h = slot_PN, slot_NP
         self.cptr_h, self.cnbr_h = cptr.astype(np.int32), cnbr.astype(np.int32)
+        self.cface_h = cface.astype(np.int32)
 
         dev = self.device
         f64 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64), device=dev)  # noqa: E731
@@ -303,6 +304,12 @@ class FVMesh:
         data[self.slot_diag_h] = diag
         np.add.at(data, self.he_slot_h, off)
         return sp.csr_matrix((data, self.indices_h, self.indptr_h), shape=(self.nc, self.nc))
+
+    def ldu_off_from_faces(self, aPN: np.ndarray, aNP: np.ndarray) -> np.ndarray:
+        """Half-edge coefficients (the LDU ``off`` array, host) from per-face aPN (row P,
+        column N) and aNP (row N, column P)."""
+        f = self.cface_h >> 1
+        return np.where(self.cface_h & 1, np.asarray(aNP)[f], np.asarray(aPN)[f])
 
     def memory_bytes(self) -> int:
         return sum(v.numel() * v.element_size() for v in vars(self).values() if isinstance(v, torch.Tensor))

@@ -3,6 +3,12 @@
 ``bicgstab_jacobi`` is the reference's native entry (``flowtrace/native/__init__.py:220-237``)
 on a device CSR matrix; ``bicgstab_ldu`` and ``pcg_ldu`` run on the face (LDU) form the
 SIMPLEC stages assemble. All return ``(x, KrylovInfo)`` like the reference.
+
+The mixed-precision forms (no reference counterpart: the reference runs one precision
+throughout, ``simple.py:77``) keep FP32 matrices and vectors inside the iterations:
+``bicgstab_ldu_mixed`` solves for the FP32 correction of an FP64 iterate, ``pcg_ldu_mixed``
+refines an FP64 pressure correction with FP32 AMG-PCG solves until the FP64 residual meets
+the tolerance.
 """
 from __future__ import annotations
 
@@ -19,6 +25,7 @@ class KrylovInfo:
     iterations: int
     residual: float  # final relative residual
     converged: bool
+    rounds: int = 0  # mixed precision: FP32 corrections applied
 
 
 class Workspace:
@@ -93,3 +100,48 @@ def pcg_csr(A, b, tol: float, maxiter: int, amg=None, project=False, ws=None):
                 nat.ptr(b.contiguous()), nat.ptr(x), float(tol), int(maxiter),
                 amg.handle if amg is not None else None, int(bool(project)), nat.ptr(work))
     return x, info
+
+
+def bicgstab_ldu_mixed(fv, ex32, off32, b32, inv32, x, tol: float, maxiter: int, relative_to_initial=True,
+                       ws=None):
+    """FP32 Jacobi-BiCGSTAB for the correction of the FP64 iterate ``x`` (updated in place);
+    the operator is (ex32, off32) in difference form, ex32 holding the row sums."""
+    n = b32.shape[0]
+    work = (ws or _ws(b32.device)).get("bicgstab", n)
+    return _run(nat.lib().ftmi_bicgstab_ldu_mixed, n, nat.ptr(ex32), nat.ptr(fv.cptr), nat.ptr(fv.cnbr),
+                nat.ptr(off32), nat.ptr(b32), nat.ptr(inv32), nat.ptr(x), float(tol), int(maxiter),
+                int(bool(relative_to_initial)), nat.ptr(work))
+
+
+def pcg_ldu_mixed(fv, diag, off, off32, b, x, tol: float, maxiter: int, amg32=None, project=False,
+                  max_refine: int = 4, ws=None):
+    """FP64 iterative refinement of ``x`` (in place; pass zeros to start from zero) with FP32
+    AMG-PCG corrections; ``off32`` is ``off`` rounded to FP32, ``amg32`` an
+    ``SmoothedAggregationAMG.to_fp32()`` twin."""
+    n = b.shape[0]
+    work = (ws or _ws(b.device)).get("pcg", n)
+    it, res, rounds = ctypes.c_int64(0), ctypes.c_double(0.0), ctypes.c_int64(0)
+    st = nat.lib().ftmi_pcg_ldu_mixed(n, nat.ptr(diag), nat.ptr(fv.cptr), nat.ptr(fv.cnbr), nat.ptr(off),
+                                      nat.ptr(off32), nat.ptr(b), nat.ptr(x), float(tol),
+                                      int(maxiter), amg32.handle if amg32 is not None else None,
+                                      int(bool(project)), int(max_refine), nat.ptr(work), ctypes.byref(it),
+                                      ctypes.byref(res), ctypes.byref(rounds), nat.stream_ptr())
+    nat.check(st, "pcg mixed")
+    return KrylovInfo(int(it.value), float(res.value), bool(st == 1), int(rounds.value))
+
+
+def pcg_ldu_f32(fv, diag32, off32, b32, x32, tol: float, maxiter: int, amg32=None, project=False, ws=None):
+    """Plain FP32 AMG-PCG from zero into ``x32``."""
+    n = b32.shape[0]
+    work = (ws or _ws(b32.device)).get("pcg", n)
+    return _run(nat.lib().ftmi_pcg_ldu_f32, n, nat.ptr(diag32), nat.ptr(fv.cptr), nat.ptr(fv.cnbr), nat.ptr(off32),
+                nat.ptr(b32), nat.ptr(x32), float(tol), int(maxiter), amg32.handle if amg32 is not None else None,
+                int(bool(project)), nat.ptr(work))
+
+
+def to_f32(x: torch.Tensor, out: torch.Tensor = None) -> torch.Tensor:
+    """FP32 copy of an FP64 device vector by the library's cast kernel."""
+    if out is None:
+        out = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+    nat.check(nat.lib().ftmi_cast_f64_f32(x.numel(), nat.ptr(x), nat.ptr(out), nat.stream_ptr()), "cast")
+    return out

@@ -19,6 +19,11 @@ fields:
 
 The host reads one block of reduced scalars per iteration (residual sums, continuity, flux
 scale) and runs the reference's residual normalisation, convergence test and stability guard.
+
+``case.precision == "mixed"`` runs steps 3-4 and the energy system in FP32 (assembly
+arithmetic, matrices and BiCGSTAB, solving for FP32 corrections of the FP64 fields) and
+step 6 as FP32 AMG-PCG corrections refined until the FP64 residual of the FP64 pressure
+system meets the tolerance; steps 1-2, 5 and 7 and all state stay FP64.
 """
 from __future__ import annotations
 
@@ -35,7 +40,7 @@ from . import _native as nat
 from .amg import SmoothedAggregationAMG
 from .case import FLUENT_TO_KIND, BoundarySpec, UnstructuredCase
 from .fvmesh import FVMesh
-from .krylov import KrylovInfo, Workspace, bicgstab_ldu, pcg_ldu
+from .krylov import KrylovInfo, Workspace, bicgstab_ldu, bicgstab_ldu_mixed, pcg_ldu, pcg_ldu_mixed, to_f32
 from .mesh import UnstructuredMesh
 
 WALL, INLET, OUTLET, PRESSURE, SYMMETRY = range(5)
@@ -78,6 +83,12 @@ class SimpleSolver:
         fv = self.fv = FVMesh(mesh, device)
         self.dim, self.nc = mesh.dim, mesh.n_cells
         self.energy = case.energy
+        # "mixed": FP32 transport assembly and solves (FP32 corrections of the FP64 fields) and
+        # FP32 AMG-PCG refined in FP64; the continuity residual, mass imbalance, pressure
+        # system and the fields stay FP64
+        self.mixed = case.precision == "mixed"
+        self.max_refine = 4
+        self._amg32 = None
         self.timers: Dict[str, float] = defaultdict(float)
         self.iteration = 0
         self.step_count = 0
@@ -171,6 +182,12 @@ class SimpleSolver:
         if self.energy:
             self.Tb, self.gT, self.Tnew, self.bT = z(nb), z(nc, dim), z(nc), z(nc)
             self.diagT, self.diagT_r = z(nc), z(nc)
+        if self.mixed:
+            f = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
+            self.diag32, self.diag_r32, self.inv_r32, self.off32 = f(nc), f(nc), f(nc), f(2 * nf)
+            self.ex32 = f(nc)
+            self.b32 = f(dim, nc)
+            self.offp32 = f(2 * nf)
         self.scal = z(_NSCAL)
         self.red_partials = z(2048 * 8)
         self.red_counter = torch.zeros(64, dtype=torch.int32, device=dev)
@@ -241,11 +258,17 @@ class SimpleSolver:
         a.cp = float(c.cp)
         a.diag, a.diag_r, a.off, a.dP = nat.ptr(self.diag), nat.ptr(self.diag_r), nat.ptr(self.off), nat.ptr(self.dP)
         a.resid, a.inv_r = nat.ptr(self.scal[_S_MOM:]), nat.ptr(self.inv_r)
+        if self.mixed:
+            self._fp32_outputs(a, dim)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "momentum assembly")
         infos = []
         for k in range(dim):
-            infos.append(bicgstab_ldu(fv, self.diag_r, self.off, self.bmom[k], self.inv_r, self.Ustar[k], 0.1, 30,
-                                      True, ws=self.ws))
+            if self.mixed:
+                infos.append(bicgstab_ldu_mixed(fv, self.ex32, self.off32, self.b32[k], self.inv_r32,
+                                                self.Ustar[k], 0.1, 30, True, ws=self.ws))
+            else:
+                infos.append(bicgstab_ldu(fv, self.diag_r, self.off, self.bmom[k], self.inv_r, self.Ustar[k], 0.1,
+                                          30, True, ws=self.ws))
         self._mark("momentum")
         # 5. Rhie-Chow fluxes and the pressure-correction system
         nat.check(L.ftmi_rhie_chow(M, B, float(rho), nat.ptr(self.Ustar), nat.ptr(self.p), nat.ptr(self.gp),
@@ -257,12 +280,19 @@ class SimpleSolver:
         if self._amg is None or self._amg_age >= self.amg_rebuild:
             t0 = time.perf_counter()
             self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
+            self._amg32 = self._amg.to_fp32() if self.mixed else None
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
             self.amg_rebuilds += 1
         self._amg_age += 1
-        pinfo = pcg_ldu(fv, self.diag_p, self.offp, self.rhs_p, self.pp, c.pressure_tol, 200, self._amg,
-                        self.singular_p, ws=self.ws)
+        if self.mixed:
+            to_f32(self.offp, self.offp32)
+            self.pp.zero_()
+            pinfo = pcg_ldu_mixed(fv, self.diag_p, self.offp, self.offp32, self.rhs_p, self.pp, c.pressure_tol, 200,
+                                  self._amg32, self.singular_p, self.max_refine, ws=self.ws)
+        else:
+            pinfo = pcg_ldu(fv, self.diag_p, self.offp, self.rhs_p, self.pp, c.pressure_tol, 200, self._amg,
+                            self.singular_p, ws=self.ws)
         self._mark("pressure")
         # 7. corrections
         nat.check(L.ftmi_correct(M, B, nat.ptr(self.pp), nat.ptr(self.Ustar), nat.ptr(self.dP), nat.ptr(self.Fst),
@@ -290,6 +320,8 @@ class SimpleSolver:
         self.step_count = self.iteration
         self.last_linear = {"momentum_iters": [i.iterations for i in infos], "pressure_iters": pinfo.iterations,
                             "pressure_residual": pinfo.residual}
+        if self.mixed:
+            self.last_linear["pressure_rounds"] = pinfo.rounds
         if tinfo is not None:
             self.last_linear["energy_iters"] = tinfo.iterations
         self._capture = {"mom_info": infos, "p_info": pinfo, "energy_info": tinfo}
@@ -323,11 +355,25 @@ class SimpleSolver:
         a.relax, a.second_order, a.simplec, a.cp = float(c.relax_T), int(c.scheme == "second-order"), -1, float(c.cp)
         a.diag, a.diag_r, a.off = nat.ptr(self.diagT), nat.ptr(self.diagT_r), nat.ptr(self.off)
         a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(self.inv_r)
+        if self.mixed:
+            self._fp32_outputs(a, 1)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
-        info = bicgstab_ldu(self.fv, self.diagT_r, self.off, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
-                            ws=self.ws)
+        if self.mixed:
+            info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
+                                      True, ws=self.ws)
+        else:
+            info = bicgstab_ldu(self.fv, self.diagT_r, self.off, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
+                                ws=self.ws)
         self.T, self.Tnew = self.Tnew, self.T
         return info
+
+    def _fp32_outputs(self, a, nfields):
+        """Point a transport assembly at the FP32 system buffers (mixed precision)."""
+        a.out_fp32 = 1
+        a.diag32, a.diag_r32, a.off32 = nat.ptr(self.diag32), nat.ptr(self.diag_r32), nat.ptr(self.off32)
+        a.inv_r32, a.ex32 = nat.ptr(self.inv_r32), nat.ptr(self.ex32)
+        for k in range(nfields):
+            a.b32[k] = nat.ptr(self.b32[k])
 
     def _accumulate_timers(self):
         evs = self._events
@@ -511,7 +557,7 @@ class SimpleSolver:
 
     def describe(self) -> dict:
         return {"solver": f"pressure-based {self.algo}", "backend": "hip", "device": str(self.device),
-                "precision": "float64", "cells": self.nc, "faces": self.mesh.n_faces, "dim": self.dim,
+                "precision": "mixed (FP32 solves, FP64 refinement)" if self.mixed else "float64", "cells": self.nc, "faces": self.mesh.n_faces, "dim": self.dim,
                 "convection": self.case.scheme, "relax_p": self.relax_p,
                 "stability_rollbacks": list(self.guard_events),
                 "stability_guard_exhausted_at": self.guard_exhausted_at,

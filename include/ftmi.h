@@ -107,6 +107,37 @@ int ftmi_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const
                  double* x, double tol, int64_t maxiter, void* amg, int project, void* work, int64_t* iters,
                  double* res, void* stream);
 
+/* ---------------------------------------------------------------- mixed precision (new; the
+ * reference runs one precision throughout, simple.py:77). FP32 matrices, vectors and
+ * arithmetic inside the Krylov iterations; FP64 where the outer iteration needs it. */
+/* FP32 PCG with an FP32 hierarchy (ftmi_amg_create_f32); work: ftmi_pcg_work_bytes(n) */
+int ftmi_pcg_ldu_f32(int64_t n, const float* diag, const int32_t* ptr, const int32_t* nbr, const float* off,
+                     const float* b, float* x, double tol, int64_t maxiter, void* amg, int project, void* work,
+                     int64_t* iters, double* res, void* stream);
+/* transport solve in correction form: rb = b - A x in FP32, Jacobi-BiCGSTAB in FP32 on
+ * A d = rb from d = 0 (the same iterates and stopping rule as the solve from x, with the
+ * absolute floor 1e-14 ||rb||), then x += d in FP64. A is applied in difference form,
+ * ex_i x_i + sum_k off_k (x_nbr - x_i), with ex the row sums (ftmi_assemble_args.ex32).
+ * work: ftmi_bicgstab_work_bytes(n) */
+int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, const int32_t* nbr, const float* off,
+                            const float* b, const float* inv_diag, double* x, double tol, int64_t maxiter,
+                            int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
+/* pressure solve by FP64 iterative refinement: r = b - A x in FP64 (projected when
+ * `project`), stop when ||r|| / ||b|| < tol, else FP32 AMG-PCG on A32 d = r to the level
+ * tol ||b||, its updates added straight into the FP64 x, at most max_refine times (FP32
+ * matrix, hierarchy and Krylov vectors; FP64 iterate). A32 is applied in difference form,
+ * ex_i d_i + sum_k off32_k (d_nbr - d_i), with ex the FP64 row sums rounded once, so FP32
+ * rounding does not swamp the residual of the nearly singular pressure Laplacian.
+ * off32 = (float) off. *iters: FP32 PCG iterations in total; *res: the final FP64 relative
+ * residual; *rounds: corrections applied. Returns 1 if the FP64 residual met tol.
+ * work: ftmi_pcg_work_bytes(n) */
+int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
+                       const float* off32, const double* b, double* x, double tol, int64_t maxiter, void* amg32,
+                       int project, int max_refine, void* work, int64_t* iters, double* res, int64_t* rounds,
+                       void* stream);
+/* y = (float) x */
+int ftmi_cast_f64_f32(int64_t n, const double* x, float* y, void* stream);
+
 /* ---------------------------------------------------------------- smoothed-aggregation AMG
  * replaces SmoothedAggregationAMG (flowtrace/linalg/amg.py:80-146). The hierarchy handle
  * holds device pointers owned by the caller; levels are added finest first; the coarsest
@@ -120,6 +151,15 @@ int ftmi_amg_add_level(void* amg, int64_t n, const int32_t* a_ptr, const int32_t
                        double* vy, double* vr);
 int ftmi_amg_set_coarse(void* amg, int64_t n, const double* inv, double* vb, double* vx);
 int ftmi_amg_vcycle(void* amg, const double* b, double* x, void* stream);
+/* the same hierarchy with FP32 values and vectors (new: the mixed-precision pressure solve);
+ * a handle takes only the entry points of its own precision */
+void* ftmi_amg_create_f32(double omega, int sweeps);
+int ftmi_amg_add_level_f32(void* amg, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const float* a_val,
+                           const int32_t* p_ptr, const int32_t* p_col, const float* p_val, const int32_t* r_ptr,
+                           const int32_t* r_col, const float* r_val, const float* dinv, float* vb, float* vx,
+                           float* vy, float* vr);
+int ftmi_amg_set_coarse_f32(void* amg, int64_t n, const float* inv, float* vb, float* vx);
+int ftmi_amg_vcycle_f32(void* amg, const float* b, float* x, void* stream);
 /* host helper (host pointers): three-pass greedy aggregation of amg.py:26-57 over a strength
  * graph in CSR form; returns the number of aggregates */
 int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* idx, int64_t* agg);
@@ -212,6 +252,12 @@ typedef struct ftmi_assemble_args {
     double* resid;            /* device, 2 nfields doubles */
     double* inv_r;            /* 1 / diag_r (may be NULL) */
     double* x0[3];            /* copies of phi[q], the Krylov start vectors (may be NULL) */
+    int out_fp32;             /* 1: FP32 arithmetic, and diag/diag_r/off/b/inv_r go to the FP32
+                                 arrays below (the mixed-precision transport systems); dP, resid
+                                 and x0 stay FP64 */
+    float *diag32, *diag_r32, *off32, *b32[3], *inv_r32;
+    float* ex32;              /* out_fp32: row sums diag_r + sum off of the relaxed operator,
+                                 built from the face fluxes (the difference form) */
 } ftmi_assemble_args;
 
 /* new: boundary velocities and pressures (simple.py:305-325); U is (dim, nc) component-major,
