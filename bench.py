@@ -144,6 +144,7 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
         s.timers[k] = 0.0
     s.amg_setup_seconds = 0.0
     s.amg_rebuilds = 0
+    s.amg_timing = {}
     guard0 = len(s.guard_events)
     pits = []
     torch.cuda.synchronize()
@@ -159,6 +160,7 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
           f"pressure iterations {pits}; last {s.last_linear}", file=sys.stderr, flush=True)
     info = {"stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
             "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
+            "amg_setup_host_seconds": {k: round(v, 4) for k, v in s.amg_timing.items()},
             "guard_events": guards, "pressure_iters_total": int(sum(p or 0 for p in pits))}
     return s, dt, info
 

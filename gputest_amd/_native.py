@@ -78,10 +78,10 @@ _SIGS = {
     "ftmi_amg_aggregate_host": ([I64, P, P, P], I64),
     "ftmi_amg_aggregate_mask_host": ([I64, P, P, P, P, P], I64),
     "ftmi_amgs_ldu_to_csr": ([I64] + [P] * 8, I32),
-    "ftmi_amgs_diag": ([I64] + [P] * 6, I32),
-    "ftmi_amgs_strength": ([I64, P, P, P, P, F64, P, P], I32),
+    "ftmi_amgs_diag": ([I64, I64] + [P] * 6, I32),
+    "ftmi_amgs_strength": ([I64, I64, P, P, P, P, F64, P, P], I32),
     "ftmi_amgs_tentative": ([I64] + [P] * 5, I32),
-    "ftmi_amgs_power": ([I64, P, P, P, P, P, P, I32, P, P, P], I32),
+    "ftmi_amgs_power": ([I64, I64, P, P, P, P, P, P, I32, P, P, P], I32),
     "ftmi_amgs_spgemm": ([I32, I64] + [P] * 10 + [F64] + [P] * 8, I32),
     "ftmi_scan_i32": ([I64, P, P, P, P], I32),
     "ftmi_amgs_count_cols": ([I64, P, P, P], I32),

@@ -167,7 +167,13 @@ class SmoothedAggregationAMG:
         return DevCSR(na, P.n, rptr, rcol, rval)
 
     def _setup(self, A: DevCSR, host_pattern):
+        import time
+
         L, st, dev = nat.lib(), nat.stream_ptr(), self.device
+        clock = time.perf_counter
+        # host seconds per phase: waiting for the strength mask, the greedy aggregation, the rest
+        self.timing = {"wait_mask": 0.0, "aggregate": 0.0, "total": 0.0}
+        t_start = clock()
         f64 = lambda *s: torch.empty(*s, dtype=torch.float64, device=dev)  # noqa: E731
         self._ovf = torch.zeros(4, dtype=torch.int32, device=dev)
         self._bsum = torch.empty(2048, dtype=torch.int64, device=dev)
@@ -177,10 +183,10 @@ class SmoothedAggregationAMG:
         while A.n > self.max_coarse and len(self.levels) < self.max_levels - 1:
             n = A.n
             diag, dinv = f64(n), f64(n)
-            nat.check(L.ftmi_amgs_diag(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag), nat.ptr(dinv),
-                                       st), "diag")
+            nat.check(L.ftmi_amgs_diag(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
+                                       nat.ptr(dinv), st), "diag")
             mask = torch.empty(A.nnz, dtype=torch.uint8, device=dev)
-            nat.check(L.ftmi_amgs_strength(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
+            nat.check(L.ftmi_amgs_strength(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
                                            float(self.theta), nat.ptr(mask), st), "strength")
             # the strength mask (and a coarse level's pattern) go to pinned host memory while
             # the GPU runs the power iteration, which does not depend on the aggregates
@@ -194,16 +200,20 @@ class SmoothedAggregationAMG:
             copied = torch.cuda.Event()
             copied.record()
             x = start_vector_device(n, dev)
-            y, rho = f64(n), f64(2)
-            nat.check(L.ftmi_amgs_power(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv), nat.ptr(x),
-                                        nat.ptr(y), 15, nat.ptr(partials), nat.ptr(rho), st), "power")
+            y, rho = f64(2 * n), f64(2)
+            nat.check(L.ftmi_amgs_power(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv),
+                                        nat.ptr(x), nat.ptr(y), 15, nat.ptr(partials), nat.ptr(rho), st), "power")
+            t0 = clock()
             copied.synchronize()
+            t1 = clock()
             ptr_h, col_h = pattern
             mask_h = mask_p.numpy()
             agg_p, size_p = _pinned(n, torch.int32), _pinned(n, torch.int32)
             agg_h, size_h = agg_p.numpy(), size_p.numpy()
             na = int(L.ftmi_amg_aggregate_mask_host(n, ptr_h.ctypes.data, col_h.ctypes.data, mask_h.ctypes.data,
                                                     agg_h.ctypes.data, size_h.ctypes.data))
+            self.timing["wait_mask"] += t1 - t0
+            self.timing["aggregate"] += clock() - t1
             if na >= n * 0.9:
                 break
             agg = torch.empty(n, dtype=torch.int32, device=dev)
@@ -237,6 +247,7 @@ class SmoothedAggregationAMG:
         self.coarse_n = Ad.shape[0]
         self.coarse_inv_h = cinv
         self._upload(cinv)
+        self.timing["total"] = clock() - t_start
 
     def _upload(self, cinv):
         dev, L = self.device, nat.lib()

@@ -43,30 +43,53 @@ __global__ void __launch_bounds__(kBlock) k_ldu_to_csr(int64_t n, const int32_t*
     }
 }
 
+// Row r is handled by TPR lanes (1 on the sparse fine levels, 64 on dense coarse levels).
+template <int TPR>
+struct LaneRows {
+    int64_t first, stride;
+    int lane;
+    __device__ LaneRows() {
+        const int g = gridDim.x, b = blockIdx.x;
+        int logical = b;
+        if (g >= kNXcd && g % kNXcd == 0) logical = (b % kNXcd) * (g / kNXcd) + b / kNXcd;
+        first = (int64_t)logical * (kBlock / TPR) + threadIdx.x / TPR;
+        stride = (int64_t)g * (kBlock / TPR);
+        lane = threadIdx.x % TPR;
+    }
+};
+
+inline int rows_tpr(int64_t n, int64_t nnz) { return n > 0 && (double)nnz / (double)n > 96.0 ? 64 : 1; }
+inline int rows_grid(int64_t n, int tpr) { return grid_for(n * tpr); }
+
+// the diagonal entry (the row's one entry with col == row, else 0)
+template <int TPR>
 __global__ void __launch_bounds__(kBlock) k_diag(int64_t n, const int32_t* __restrict__ ptr,
                                                  const int32_t* __restrict__ col, const double* __restrict__ val,
                                                  double* __restrict__ diag, double* __restrict__ dinv) {
-    int64_t lo, hi;
-    block_range(n, lo, hi);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    LaneRows<TPR> R;
+    for (int64_t i = R.first; i < n; i += R.stride) {
         double d = 0.0;
-        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j)
+        for (int32_t j = ptr[i] + R.lane; j < ptr[i + 1]; j += TPR)
             if (col[j] == i) d += val[j];
-        diag[i] = d;
-        dinv[i] = 1.0 / d;
+#pragma unroll
+        for (int off = TPR / 2; off > 0; off >>= 1) d += __shfl_xor(d, off, TPR);
+        if (R.lane == 0) {
+            diag[i] = d;
+            dinv[i] = 1.0 / d;
+        }
     }
 }
 
+template <int TPR>
 __global__ void __launch_bounds__(kBlock) k_strength(int64_t n, const int32_t* __restrict__ ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val,
                                                      const double* __restrict__ diag, double theta,
                                                      uint8_t* __restrict__ mask) {
-    int64_t lo, hi;
-    block_range(n, lo, hi);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    LaneRows<TPR> R;
+    for (int64_t i = R.first; i < n; i += R.stride) {
         const double di = fabs(diag[i]);
-        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) {
+        for (int32_t j = ptr[i] + R.lane; j < ptr[i + 1]; j += TPR) {
             const int32_t c = col[j];
             mask[j] = (c != i && fabs(val[j]) >= theta * sqrt(di * fabs(diag[c]))) ? 1 : 0;
         }
@@ -86,42 +109,57 @@ __global__ void __launch_bounds__(kBlock) k_tentative(int64_t n, const int32_t* 
 }
 
 // ---------------------------------------------------------------- power iteration for rho(D^-1 A)
-__global__ void __launch_bounds__(kBlock) k_pow_spmv(int64_t n, const int32_t* __restrict__ ptr,
-                                                     const int32_t* __restrict__ col,
-                                                     const double* __restrict__ val,
-                                                     const double* __restrict__ dinv,
-                                                     const double* __restrict__ x, double* __restrict__ y,
-                                                     double* partials) {
-    int64_t lo, hi;
-    block_range(n, lo, hi);
+// One launch per iteration: the previous iterate is normalised on the fly (x = y_prev /
+// max(||y_prev||, 1e-300), ||y_prev|| re-added from the previous partials by every
+// workgroup), then y = (D^-1 A) x with the products (d_i a_ij) x_j of a row-by-row sparse
+// product, and the partials of ||y||^2 and ||x||^2. The first iteration takes x as given.
+// Rows are handled by TPR lanes: 1 keeps the column-order sum, wider groups (dense coarse
+// levels only) sum in a fixed lane tree.
+template <int TPR>
+__global__ void __launch_bounds__(kBlock) k_pow_step(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dinv, const double* __restrict__ y_prev,
+                                                     double* __restrict__ x, double* __restrict__ y, int first,
+                                                     const double* part_in, int np_in, double* part_out) {
+    double scale = 1.0;
+    if (!first) {
+        double tot[2];
+        consume_sum<2>(part_in, np_in, tot);
+        const double ny = sqrt(tot[0]);
+        scale = ny > 1e-300 ? ny : 1e-300;
+    }
+    auto xv = [&](int64_t j) { return first ? x[j] : y_prev[j] / scale; };
+    const int g = gridDim.x, b = blockIdx.x;
+    int logical = b;
+    if (g >= kNXcd && g % kNXcd == 0) logical = (b % kNXcd) * (g / kNXcd) + b / kNXcd;
+    const int64_t stride = (int64_t)g * (kBlock / TPR);
+    const int lane = threadIdx.x % TPR;
     double acc[2] = {0.0, 0.0};
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    for (int64_t i = (int64_t)logical * (kBlock / TPR) + threadIdx.x / TPR; i < n; i += stride) {
         const double di = dinv[i];
         double s = 0.0;
-        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) s += (di * val[j]) * x[col[j]];
-        y[i] = s;
-        acc[0] += s * s;
-        acc[1] += x[i] * x[i];
+        const int32_t e = ptr[i + 1];
+        for (int32_t j = ptr[i] + lane; j < e; j += TPR) s += (di * val[j]) * xv(col[j]);
+#pragma unroll
+        for (int off = TPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, TPR);
+        if (lane == 0) {
+            const double xi = xv(i);
+            if (!first) x[i] = xi;
+            y[i] = s;
+            acc[0] += s * s;
+            acc[1] += xi * xi;
+        }
     }
-    block_partials<2>(acc, partials);
+    block_partials<2>(acc, part_out);
 }
 
-// lam = ||y|| / max(||x||, 1e-300); scale[0] = max(||y||, 1e-300)
-__global__ void __launch_bounds__(kBlock) k_pow_fin(const double* partials, int nparts, double* lam, double* scale) {
+// lam = ||y|| / max(||x||, 1e-300)
+__global__ void __launch_bounds__(kBlock) k_pow_fin(const double* partials, int nparts, double* lam) {
     double tot[2];
     if (finalize_sum<2>(partials, nparts, tot)) {
         const double ny = sqrt(tot[0]), nx = sqrt(tot[1]);
         *lam = ny / (nx > 1e-300 ? nx : 1e-300);
-        *scale = ny > 1e-300 ? ny : 1e-300;
     }
-}
-
-__global__ void __launch_bounds__(kBlock) k_pow_scale(int64_t n, const double* __restrict__ y, double* __restrict__ x,
-                                                      const double* scale) {
-    const double s = *scale;
-    int64_t lo, hi;
-    block_range(n, lo, hi);
-    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) x[i] = y[i] / s;
 }
 
 // ---------------------------------------------------------------- wave-per-row sparse products
@@ -455,16 +493,24 @@ int ftmi_amgs_ldu_to_csr(int64_t n, const int32_t* cptr, const int32_t* he_slot,
     return 0;
 }
 
-int ftmi_amgs_diag(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, double* diag, double* dinv,
-                   void* stream) {
-    k_diag<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, ptr, col, val, diag, dinv);
+int ftmi_amgs_diag(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val, double* diag,
+                   double* dinv, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (rows_tpr(n, nnz) == 64)
+        k_diag<64><<<rows_grid(n, 64), kBlock, 0, s>>>(n, ptr, col, val, diag, dinv);
+    else
+        k_diag<1><<<rows_grid(n, 1), kBlock, 0, s>>>(n, ptr, col, val, diag, dinv);
     FTMI_LAUNCH_CHECK("ftmi_amgs_diag");
     return 0;
 }
 
-int ftmi_amgs_strength(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* diag,
-                       double theta, uint8_t* mask, void* stream) {
-    k_strength<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, ptr, col, val, diag, theta, mask);
+int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
+                       const double* diag, double theta, uint8_t* mask, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (rows_tpr(n, nnz) == 64)
+        k_strength<64><<<rows_grid(n, 64), kBlock, 0, s>>>(n, ptr, col, val, diag, theta, mask);
+    else
+        k_strength<1><<<rows_grid(n, 1), kBlock, 0, s>>>(n, ptr, col, val, diag, theta, mask);
     FTMI_LAUNCH_CHECK("ftmi_amgs_strength");
     return 0;
 }
@@ -476,17 +522,28 @@ int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, 
     return 0;
 }
 
-// rho(D^-1 A) by `iters` power iterations from x (overwritten); y: n doubles of work;
-// partials: >= 2048 * 2 doubles; out[0] = lam (device), out[1] used as scratch.
-int ftmi_amgs_power(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* dinv,
-                    double* x, double* y, int iters, double* partials, double* out, void* stream) {
+// rho(D^-1 A) by `iters` power iterations from x (overwritten); y: 2 n doubles of work;
+// partials: >= 2 * 1024 * 2 doubles; out[0] = lam (device).
+int ftmi_amgs_power(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
+                    const double* dinv, double* x, double* y, int iters, double* partials, double* out, void* stream) {
+    if (n <= 0 || iters <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    const int g = grid_for(n);
+    const double avg = (double)nnz / (double)n;
+    const int tpr = avg > 96.0 ? 64 : 1;  // wide rows only on the dense coarse levels
+    int64_t g64 = (n * tpr + kBlock - 1) / kBlock;
+    const int g = (int)(g64 < 1 ? 1 : (g64 > kRedGrid ? kRedGrid : g64));
+    double* part[2] = {partials, partials + 2 * kRedGrid};
+    double* yb[2] = {y, y + n};
     for (int it = 0; it < iters; ++it) {
-        k_pow_spmv<<<g, kBlock, 0, s>>>(n, ptr, col, val, dinv, x, y, partials);
-        k_pow_fin<<<1, kBlock, 0, s>>>(partials, g, &out[0], &out[1]);
-        k_pow_scale<<<g, kBlock, 0, s>>>(n, y, x, &out[1]);
+        const int a = it & 1, c = a ^ 1;
+        if (tpr == 64)
+            k_pow_step<64><<<g, kBlock, 0, s>>>(n, ptr, col, val, dinv, yb[c], x, yb[a], it == 0, part[c], g,
+                                                part[a]);
+        else
+            k_pow_step<1><<<g, kBlock, 0, s>>>(n, ptr, col, val, dinv, yb[c], x, yb[a], it == 0, part[c], g,
+                                               part[a]);
     }
+    k_pow_fin<<<1, kBlock, 0, s>>>(part[(iters - 1) & 1], g, &out[0]);
     FTMI_LAUNCH_CHECK("ftmi_amgs_power");
     return 0;
 }

@@ -1,5 +1,6 @@
 // Host-side set-up helpers of libftmi (plain C++; called once per mesh or per AMG rebuild).
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "../../include/ftmi.h"
@@ -73,34 +74,55 @@ int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* id
 
 // The same aggregation over a CSR pattern and a strength mask (mask[j] != 0: entry j is a
 // strong off-diagonal connection), int32 in and out; size[a] receives each aggregate's size.
+// Pass 2 (join the first aggregated strong neighbour, in order), the collection of the nodes
+// left for pass 3 and the aggregate sizes share one scan. (Pass 1, the greedy choice of roots,
+// is inherently ordered: a block-parallel replay does not re-synchronise with the sequential
+// tiling on structured orderings, so it stays one thread.)
 int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
                                      int32_t* agg, int32_t* size) {
-    int64_t na = 0;
-    for (int64_t i = 0; i < n; ++i) agg[i] = -1;
+    if (n <= 0) return 0;
+    std::memset(agg, 0xff, (size_t)n * sizeof(int32_t));  // -1
+    int32_t na = 0;
     for (int64_t i = 0; i < n; ++i) {
         if (agg[i] >= 0) continue;
+        const int32_t e = ptr[i + 1];
         bool all_free = true;
-        for (int32_t k = ptr[i]; k < ptr[i + 1] && all_free; ++k)
-            if (mask[k]) all_free = agg[col[k]] < 0;
-        if (!all_free) continue;
-        agg[i] = (int32_t)na;
-        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k)
-            if (mask[k]) agg[col[k]] = (int32_t)na;
-        ++na;
-    }
-    for (int64_t i = 0; i < n; ++i) {
-        if (agg[i] >= 0) continue;
-        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        for (int32_t k = ptr[i]; k < e; ++k) {
             if (mask[k] && agg[col[k]] >= 0) {
-                agg[i] = agg[col[k]];
+                all_free = false;
                 break;
             }
         }
+        if (!all_free) continue;
+        agg[i] = na;
+        for (int32_t k = ptr[i]; k < e; ++k)
+            if (mask[k]) agg[col[k]] = na;
+        size[na] = 0;
+        ++na;
     }
-    for (int64_t i = 0; i < n; ++i)
-        if (agg[i] < 0) agg[i] = (int32_t)na++;
-    for (int64_t a = 0; a < na; ++a) size[a] = 0;
-    for (int64_t i = 0; i < n; ++i) ++size[agg[i]];
+    std::vector<int32_t> left;
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t a = agg[i];
+        if (a < 0) {
+            for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+                if (mask[k] && agg[col[k]] >= 0) {
+                    a = agg[col[k]];
+                    break;
+                }
+            }
+            if (a < 0) {
+                left.push_back((int32_t)i);
+                continue;
+            }
+            agg[i] = a;
+        }
+        ++size[a];
+    }
+    for (const int32_t i : left) {
+        agg[i] = na;
+        size[na] = 1;
+        ++na;
+    }
     return na;
 }
 

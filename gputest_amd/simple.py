@@ -99,6 +99,7 @@ class SimpleSolver:
         self.amg_rebuild = 10
         self.amg_setup_seconds = 0.0
         self.amg_rebuilds = 0
+        self.amg_timing: Dict[str, float] = {}
         self.last_linear = {}
         self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
         self._build_boundaries()
@@ -283,6 +284,8 @@ class SimpleSolver:
             self._amg32 = self._amg.to_fp32() if self.mixed else None
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
+            for k, v in self._amg.timing.items():
+                self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
             self.amg_rebuilds += 1
         self._amg_age += 1
         if self.mixed:

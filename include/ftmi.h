@@ -173,17 +173,18 @@ int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_
 int ftmi_amgs_ldu_to_csr(int64_t n, const int32_t* cptr, const int32_t* he_slot, const int32_t* slot_diag,
                          const int32_t* indptr, const double* diag, const double* off, double* data, void* stream);
 /* diagonal and its inverse */
-int ftmi_amgs_diag(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, double* diag, double* dinv,
-                   void* stream);
+int ftmi_amgs_diag(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val, double* diag,
+                   double* dinv, void* stream);
 /* strength mask |a_ij| >= theta sqrt(|a_ii a_jj|), i != j (amg.py:60-66) */
-int ftmi_amgs_strength(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* diag,
-                       double theta, uint8_t* mask, void* stream);
+int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
+                       const double* diag, double theta, uint8_t* mask, void* stream);
 /* tentative prolongator as CSR: t_ptr = 0..n, t_val = 1/sqrt(|aggregate|) */
 int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, int32_t* t_ptr, double* t_val,
                         void* stream);
-/* rho(D^-1 A) by power iteration from x (amg.py:69-77); out[0] = the estimate (device) */
-int ftmi_amgs_power(int64_t n, const int32_t* ptr, const int32_t* col, const double* val, const double* dinv,
-                    double* x, double* y, int iters, double* partials, double* out, void* stream);
+/* rho(D^-1 A) by power iteration from x (amg.py:69-77); y: 2 n doubles of work; partials:
+ * 4096 doubles; out[0] = the estimate (device) */
+int ftmi_amgs_power(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
+                    const double* dinv, double* x, double* y, int iters, double* partials, double* out, void* stream);
 /* deterministic sparse product C = diag(xscale) X Y, count (fill = 0) or fill pass; with agg:
  * C = T - (omega_scale / *omega_num) diag(xscale) X Y (the smoothed prolongator). Columns come
  * out sorted and each entry is summed in the order of X's row. long_rows (nrows int32) and
