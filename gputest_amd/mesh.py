@@ -444,3 +444,111 @@ def channel_mesh(nx: int = 128, ny: int = 64, nz: int = 64, stretch: float = 2.0
     m2 = cartesian_mesh_2d(nx, ny, names=("left", "right", "bottom", "top"), xs=np.linspace(0.0, 4.0, nx + 1),
                            ys=tanh_nodes(ny, stretch))
     return make_periodic(extrude_to_3d(m2, nz, 1.0), "left", "right")
+
+
+def extract_cells(mesh: UnstructuredMesh, keep: np.ndarray, name: str = "body",
+                  type: str = "wall") -> UnstructuredMesh:
+    """The sub-mesh of the cells where ``keep`` is True (``unstructured.py:714-780``, 2-D or
+    3-D, no periodic pairs). Faces between a kept and a dropped cell become boundary faces
+    of a new zone ``name``, oriented out of the kept cell; faces of dropped cells only go.
+    Cells keep their order; faces are regrouped by zone (interior first)."""
+    keep = np.asarray(keep, dtype=bool)
+    if mesh.periodic:
+        raise NotImplementedError("extracting cells from a mesh with periodic pairs is not supported")
+    new_id = np.full(mesh.n_cells, -1, dtype=np.int64)
+    new_id[keep] = np.arange(int(keep.sum()))
+    own, nei = mesh.owner, mesh.neighbour
+    ko = keep[own]
+    kn = np.where(nei >= 0, keep[np.maximum(nei, 0)], False)
+    use = ko | kn
+    iface = (ko ^ kn) & (nei >= 0)  # exactly one side kept: becomes a boundary face
+    flip = iface & ~ko  # the kept cell is the neighbour: reverse the face
+    counts = np.diff(mesh.face_ptr)
+    f_idx = np.flatnonzero(use)
+    nodes_of = [mesh.face_nodes[mesh.face_ptr[f]:mesh.face_ptr[f + 1]] for f in f_idx]
+    fn = [nd[::-1] if flip[f] else nd for f, nd in zip(f_idx, nodes_of)]
+    new_own = np.where(flip[f_idx], new_id[np.maximum(nei[f_idx], 0)], new_id[own[f_idx]])
+    new_nei = np.where(iface[f_idx], -1, np.where(nei[f_idx] >= 0, new_id[np.maximum(nei[f_idx], 0)], -1))
+    zid = max(mesh.zones) + 1
+    fz = np.where(iface[f_idx], zid, mesh.face_zone[f_idx])
+    zones = {k: z for k, z in mesh.zones.items() if z.kind == "face" and np.any(fz == k)}
+    zones[zid] = Zone(zid, name, type, "face")
+    kept_cz = mesh.cell_zone[keep]
+    for k, z in mesh.zones.items():
+        if z.kind == "cell" and np.any(kept_cz == k):
+            w = np.flatnonzero(kept_cz == k)
+            zones[k] = Zone(k, z.name, z.type, "cell", int(w[0]), int(w[-1]))
+    ptr = np.zeros(f_idx.size + 1, dtype=np.int64)
+    np.cumsum(counts[f_idx], out=ptr[1:])
+    sub = UnstructuredMesh(dim=mesh.dim, nodes=mesh.nodes, face_ptr=ptr,
+                           face_nodes=np.concatenate(fn).astype(np.int64), owner=new_own.astype(np.int64),
+                           neighbour=new_nei.astype(np.int64), face_zone=fz.astype(np.int64),
+                           n_cells=int(keep.sum()), cell_zone=kept_cz.copy(), zones=zones,
+                           name=f"{mesh.name}-{name}")
+    return sort_faces_by_zone(sub)
+
+
+def graded_nodes(a: float, b: float, h_near: float, n: int, toward: str) -> np.ndarray:
+    """n+1 nodes on [a, b], geometrically graded to spacing h_near at the ``toward`` end
+    ("a" or "b"), with the growth ratio chosen so the n cells span b - a exactly."""
+    L = b - a
+    if n * h_near >= L:
+        return np.linspace(a, b, n + 1)
+    lo, hi = 1.0, 2.0
+    while h_near * (hi ** n - 1.0) / (hi - 1.0) < L:
+        hi *= 2.0
+    for _ in range(200):  # bisection on the ratio r: h (r^n - 1) / (r - 1) = L
+        r = 0.5 * (lo + hi)
+        if h_near * (r ** n - 1.0) / (r - 1.0) < L:
+            lo = r
+        else:
+            hi = r
+    r = 0.5 * (lo + hi)
+    steps = h_near * r ** np.arange(n)
+    steps *= L / steps.sum()
+    x = a + np.concatenate([[0.0], np.cumsum(steps)])
+    x[-1] = b
+    return x if toward == "a" else (a + b) - x[::-1]
+
+
+def square_cylinder_mesh(n_body: int = 40, upstream: float = 5.0, downstream: float = 15.0,
+                         half_height: float = 5.0, growth_cells: int = None, jitter: float = 0.15,
+                         seed: int = 0) -> UnstructuredMesh:
+    """Square cylinder of side D = 1 centred at the origin in a (upstream + downstream) x
+    2 half_height channel, meshed with triangles graded toward the body: n_body cells per
+    side on the body, geometric growth away from it (``growth_cells`` cells per outer segment,
+    default scaled with n_body), interior nodes off the body jittered by U(-jitter, jitter)
+    times the LOCAL spacing (seeded). Zones: inlet (velocity-inlet, x = -upstream), outlet
+    (pressure-outlet), bottom/top (symmetry), cylinder (wall). 20D x 10D with n_body = 200
+    gives about 0.5M triangles."""
+    h = 1.0 / n_body
+    m = growth_cells if growth_cells is not None else max(8, int(round(n_body * 1.6)))
+    xs = np.concatenate([graded_nodes(-upstream, -0.5, h, m, "b")[:-1], np.linspace(-0.5, 0.5, n_body + 1),
+                         graded_nodes(0.5, downstream, h, 2 * m, "a")[1:]])
+    ys = np.concatenate([graded_nodes(-half_height, -0.5, h, m, "b")[:-1], np.linspace(-0.5, 0.5, n_body + 1),
+                         graded_nodes(0.5, half_height, h, m, "a")[1:]])
+    nx, ny = xs.size - 1, ys.size - 1
+    base = cartesian_mesh_2d(nx, ny, names=("inlet", "outlet", "bottom", "top"),
+                             types=("velocity-inlet", "pressure-outlet", "symmetry", "symmetry"),
+                             triangulate=True, xs=xs, ys=ys)
+    nodes = base.nodes.copy()
+    if jitter:
+        rng = np.random.default_rng(seed)
+        X, Y = nodes[:, 0].reshape(nx + 1, ny + 1), nodes[:, 1].reshape(nx + 1, ny + 1)
+        hx = np.minimum(np.diff(xs)[:-1], np.diff(xs)[1:])  # local spacing at interior nodes
+        hy = np.minimum(np.diff(ys)[:-1], np.diff(ys)[1:])
+        jx = rng.uniform(-jitter, jitter, (nx - 1, ny - 1)) * hx[:, None]
+        jy = rng.uniform(-jitter, jitter, (nx - 1, ny - 1)) * hy[None, :]
+        xi, yi = X[1:-1, 1:-1], Y[1:-1, 1:-1]
+        on_body = (np.abs(xi) <= 0.5 + 1e-12) & (np.abs(yi) <= 0.5 + 1e-12)
+        X[1:-1, 1:-1] = np.where(on_body, xi, xi + jx)
+        Y[1:-1, 1:-1] = np.where(on_body, yi, yi + jy)
+        nodes = np.stack([X.ravel(), Y.ravel()], axis=1)
+    base.nodes = nodes
+    # a triangle is inside the body iff its three nodes are (body nodes are not jittered)
+    ptr, loops = base.cell_polygons_2d()
+    pts = nodes[loops].reshape(-1, 3, 2)
+    inside = np.all((np.abs(pts[:, :, 0]) <= 0.5 + 1e-12) & (np.abs(pts[:, :, 1]) <= 0.5 + 1e-12), axis=1)
+    mesh = extract_cells(base, ~inside, name="cylinder", type="wall")
+    mesh.name = f"square-cylinder-{n_body}"
+    return mesh

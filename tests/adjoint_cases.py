@@ -1,0 +1,24 @@
+"""Square-cylinder cases for the adjoint tests (shared by the CPU and GPU tests)."""
+import numpy as np
+
+from gputest_amd.adjoint import InletProfile
+from gputest_amd.mesh import square_cylinder_mesh
+
+HALF = 3.0
+
+
+def small_mesh():
+    return square_cylinder_mesh(4, upstream=3.0, downstream=6.0, half_height=HALF)
+
+
+def u_ctrl0():
+    return 1.0 + 0.1 * np.sin(np.linspace(0.0, 3.0, 64))
+
+
+def case(mod, u, nu, tol=1e-14, max_iter=6000):
+    """Inlet profile u (64 control values), symmetry sides, outlet, cylinder wall; Re = 1/nu."""
+    prof = InletProfile(-HALF, HALF, u)
+    B = mod.Boundary
+    c = mod.Case(boundaries={"inlet": B("inlet", profile=prof), "outlet": B("outlet"), "bottom": B("symmetry"),
+                             "top": B("symmetry"), "cylinder": B("wall")}, viscosity=nu, max_iter=max_iter, tol=tol)
+    return c, prof

@@ -1,0 +1,84 @@
+"""The discrete adjoint on the GPU (HIP forward solver, transposed momentum BiCGSTAB and
+pressure AMG-PCG in the adjoint products) against the CPU reference adjoint on the oracle's
+forward, and against central finite differences of the GPU forward; the autograd wrapper."""
+import numpy as np
+import pytest
+import torch
+
+import oracle.simplec_oracle as O
+from oracle.adjoint_oracle import CpuResidualMap, solver_view
+from gputest_amd import case as C
+from gputest_amd.adjoint import AdjointDrag, ResidualMap, drag_gradient
+from gputest_amd.simple import SimpleSolver
+from tests.adjoint_cases import case, small_mesh, u_ctrl0
+
+pytestmark = pytest.mark.gpu
+NU = 1.0 / 40.0
+DEV = "cuda"
+
+
+def _gpu(mesh, u, nu):
+    c, prof = case(C, u, nu)
+    s = SimpleSolver(mesh, c, DEV)
+    while not s.finished:
+        s.step()
+    assert s.converged
+    return s, prof
+
+
+def test_gpu_adjoint_matches_cpu_adjoint_and_central_differences():
+    mesh, u0 = small_mesh(), u_ctrl0()
+    s, prof = _gpu(mesh, u0, NU)
+    gu, gnu, J, info = drag_gradient(s, prof)
+    print("gpu adjoint", info, "J", J)  # noqa: T201
+    assert info.residual < 1e-9
+    c, prof_o = case(O, u0, NU)
+    o = O.Simplec(mesh, c)
+    while not o.finished:
+        o.step()
+    gu_c, gnu_c, J_c, info_c = drag_gradient(None, prof_o, rmap=CpuResidualMap(solver_view(o), "cylinder", "inlet",
+                                                                                 prof_o), tol=1e-11)
+    print("cpu adjoint", info_c, "J", J_c)
+    assert abs(J - J_c) <= 1e-10 * abs(J_c)
+    ref = np.r_[gu_c.numpy(), float(gnu_c)]
+    got = np.r_[gu.cpu().numpy(), float(gnu)]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("gpu vs cpu adjoint", err)
+    assert err < 1e-7
+    h = 1e-6
+
+    def drag(u, nu):
+        si, pi = _gpu(mesh, u, nu)
+        rm = ResidualMap(si, "cylinder", "inlet", pi)
+        return float(rm.objective(rm.state(), rm.params()).detach())
+
+    k = 31
+    up, um = u0.copy(), u0.copy()
+    up[k] += h
+    um[k] -= h
+    fd = (drag(up, NU) - drag(um, NU)) / (2 * h)
+    print("u", k, float(gu[k]), fd)
+    assert abs(float(gu[k]) - fd) <= 1e-5 * abs(fd)
+    fd = (drag(u0, NU * (1 + h)) - drag(u0, NU * (1 - h))) / (2 * h * NU)
+    print("nu", float(gnu), fd)
+    assert abs(float(gnu) - fd) <= 1e-5 * abs(fd)
+
+
+def test_autograd_function_returns_the_adjoint_gradient():
+    mesh = small_mesh()
+    holder = {}
+
+    def make_solver(u, nu):
+        c, prof = case(C, u, nu)
+        holder["prof"] = prof
+        return SimpleSolver(mesh, c, DEV), prof
+
+    u = torch.tensor(u_ctrl0(), dtype=torch.float64, device=DEV, requires_grad=True)
+    nu = torch.tensor(NU, dtype=torch.float64, device=DEV, requires_grad=True)
+    J = AdjointDrag.apply(u, nu, make_solver)
+    (2.0 * J).backward()
+    s, prof = _gpu(mesh, u_ctrl0(), NU)
+    gu, gnu, J2, _ = drag_gradient(s, prof)
+    assert abs(float(J) - J2) <= 1e-12 * abs(J2)
+    assert torch.allclose(u.grad, 2.0 * gu, rtol=1e-8, atol=0)
+    assert abs(float(nu.grad) - 2.0 * float(gnu)) <= 1e-8 * abs(float(gnu))
