@@ -54,6 +54,8 @@ _SIGS = {
     "ftmi_gg_gradient": ([I64, I64] + [P] * 17, I32),
     "ftmi_assemble": ([I64, I64, I64] + [P] * 9, I32),
     "ftmi_assemble_plan_host": ([I64, I64, P, P, P, P, P], I32),
+    "ftmi_capture_mode": ([I32], I32),
+    "ftmi_set_grid_cap": ([I32], I32),
     "ftmi_timer_enable": ([I32], I32),
     "ftmi_timer_read": ([P, P], I32),
     "ftmi_bicgstab_work_bytes": ([I64], I64),

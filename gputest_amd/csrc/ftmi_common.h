@@ -30,10 +30,15 @@ struct RedBuf {
     unsigned* counter;
 };
 
+// Workgroup cap of the streaming kernels (kMaxGrid by default; the autotuner may change it
+// through ftmi_set_grid_cap). Reductions stay within kRedGrid partials whatever the cap.
+int grid_cap();
+
 inline int grid_for(int64_t n) {
     int64_t g = (n + kBlock - 1) / kBlock;
     if (g < 1) g = 1;
-    if (g > kMaxGrid) g = kMaxGrid;
+    const int cap = grid_cap();
+    if (g > cap) g = cap;
     return (int)g;
 }
 

@@ -114,6 +114,12 @@ struct KTimer {
 };
 static KTimer g_timer;
 
+// Capture mode: the solvers never read the device state back. Every iteration up to maxiter
+// is enqueued (kernels of a finished solve return at once on the device `done` flag) and the
+// final fix-ups read the state on the device, so a solve can be recorded in a HIP graph.
+// The iteration counts and residuals are then not reported (*iters = -1, *res = NaN).
+static bool g_capture = false;
+
 // ------------------------------------------------------------------ BiCGSTAB kernels
 // One iteration is four launches: p-update fused into v = A phat (phat recomputed at the
 // neighbours from r, p, v of the previous iteration), s-update fused into t = A shat, the
@@ -336,6 +342,21 @@ __global__ void __launch_bounds__(kBlock) bi_final(int64_t n, T* __restrict__ x0
     }
 }
 
+// bi_final with cur / fixup / alpha read from the state block on the device
+template <typename T>
+__global__ void __launch_bounds__(kBlock) bi_final_dev(int64_t n, T* __restrict__ x0, const T* __restrict__ x1,
+                                                       const T* __restrict__ ph, const State* S) {
+    const int cur = S->st[I_CUR], fixup = S->st[I_FIXUP];
+    if (!cur && !fixup) return;
+    const T alpha = (T)S->sc[S_ALPHA];
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        const T xc = cur ? x1[i] : x0[i];
+        x0[i] = fixup ? xc + alpha * ph[i] : xc;
+    }
+}
+
 template <typename T>
 struct BiWork {
     T *r, *rh, *p[2], *v[2], *ph, *s, *sh, *t, *x1;
@@ -408,6 +429,23 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
         bi_init<Op, T, false><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
     fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel);
     FTMI_LAUNCH_CHECK("bicgstab init");
+    if (g_capture) {
+        for (int64_t k = 1; k <= maxiter; ++k) {
+            const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+            bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
+                                               W.part[0]);
+            bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
+                                               (int)k);
+            bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
+                                          tol, (int)k);
+            fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter);
+        }
+        bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, W.S);
+        FTMI_LAUNCH_CHECK("bicgstab (capture)");
+        *iters = -1;
+        *res = __builtin_nan("");
+        return 0;
+    }
     if (post_state(pn, 0, W.S, st)) return -1;
     // slot `last` holds the most recently posted state; the loop waits one iteration behind
     int last = 0;
@@ -472,6 +510,12 @@ int ftmi_timer_enable(int on) {
     g_timer.total_ms = 0.0;
     g_timer.launches = 0;
     return 0;
+}
+
+int ftmi_capture_mode(int on) {
+    const int was = g_capture ? 1 : 0;
+    g_capture = on != 0;
+    return was;
 }
 
 int ftmi_timer_read(double* total_ms, int64_t* launches) {
@@ -1183,6 +1227,23 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
     else
         pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
     FTMI_LAUNCH_CHECK("pcg init");
+    if (g_capture) {
+        for (int64_t k = 1; k <= maxiter; ++k) {
+            const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+            pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
+            pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
+            fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k);
+            nz = precond();
+            if (project)
+                pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+            else
+                pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+        }
+        FTMI_LAUNCH_CHECK("pcg (capture)");
+        *iters = -1;
+        *res = __builtin_nan("");
+        return 0;
+    }
     if (post_state(pn, 0, W.S, st)) return -1;
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
@@ -1386,6 +1447,21 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
     double bnorm = 1.0, rel = 0.0;
     int conv = 0;
     *rounds = 0;
+    if (g_capture) {
+        // one correction, with the inner tolerance relative to the projected residual
+        k_resid_f64<<<g, kBlock, 0, st>>>(n, A64, b, x, r64, pa);
+        k_project_round<<<g, kBlock, 0, st>>>(n, r64, pa, g, project, r32, pb);
+        FTMI_LAUNCH_CHECK("ftmi_pcg_ldu_mixed residual (capture)");
+        int64_t it = 0;
+        double r_in = 0.0;
+        const int c = pcg_run<LduDiffOp<float>, float, double>(n, A32, r32, x, tol, maxiter, M, project, inner, &it,
+                                                               &r_in, st, false);
+        if (c < 0) return c;
+        *iters = -1;
+        *res = __builtin_nan("");
+        *rounds = 1;
+        return 0;
+    }
     for (int round = 0; round <= max_refine; ++round) {
         // the FP64 residual of the current iterate, projected, and its norm (one host read)
         k_resid_f64<<<g, kBlock, 0, st>>>(n, A64, b, x, r64, pa);

@@ -15,6 +15,8 @@ void set_error(const char* where, hipError_t e) {
     snprintf(g_err, sizeof(g_err), "%s: %s", where, hipGetErrorString(e));
 }
 void set_error_msg(const char* msg) { snprintf(g_err, sizeof(g_err), "%s", msg); }
+static int g_grid_cap = kMaxGrid;
+int grid_cap() { return g_grid_cap; }
 }  // namespace ftmi
 
 extern "C" {
@@ -22,6 +24,12 @@ extern "C" {
 int ftmi_abi_version(void) { return FTMI_ABI_VERSION; }
 
 const char* ftmi_last_error(void) { return ftmi::g_err; }
+
+int ftmi_set_grid_cap(int cap) {
+    const int was = ftmi::g_grid_cap;
+    if (cap >= 64 && cap <= 65536) ftmi::g_grid_cap = cap;
+    return was;
+}
 
 int ftmi_device_count(void) {
     int n = 0;

@@ -87,6 +87,11 @@ class SimpleSolver:
         # FP32 AMG-PCG refined in FP64; the continuity residual, mass imbalance, pressure
         # system and the fields stay FP64
         self.mixed = case.precision == "mixed"
+        # replay each outer iteration from a HIP graph (see _graph_step)
+        self.use_graph = False
+        self._graph = None
+        self.graph_captures = 0
+        self._pin = None
         self.max_refine = 4
         self._amg32 = None
         self.timers: Dict[str, float] = defaultdict(float)
@@ -211,6 +216,8 @@ class SimpleSolver:
         return self.converged or self.iteration >= self.case.max_iter
 
     def _mark(self, name):
+        if self.use_graph:
+            return
         ev = torch.cuda.Event(enable_timing=True)
         ev.record()
         self._events.append((name, ev))
@@ -218,12 +225,46 @@ class SimpleSolver:
     def step(self) -> float:
         if self.finished:
             return 0.0
+        self._events = []
+        if self.use_graph:
+            return self._graph_step()
+        self._mark("start")
+        out = self._launch(rebuild=True)
+        self._amg_age += 1
+        return self._finish(*out)
+
+    def _graph_step(self) -> float:
+        """One outer iteration replayed from a HIP graph of the device work. The graph is
+        recorded (in capture mode: Krylov solves without host reads) after each eager step that
+        rebuilds the AMG hierarchy, and replayed until the next rebuild; the host logic
+        (residual normalisation, convergence, guard) runs after every replay."""
+        due = self._amg is None or self._amg_age >= self.amg_rebuild or not self._has_lag
+        if due:
+            self._graph = None
+            out = self._launch(rebuild=True)
+        else:
+            if self._graph is None:
+                g = torch.cuda.CUDAGraph()
+                L = nat.lib()
+                prev = L.ftmi_capture_mode(1)
+                try:
+                    with torch.cuda.graph(g):
+                        self._graph_out = self._launch(rebuild=False)
+                finally:
+                    L.ftmi_capture_mode(prev)
+                self._graph = g
+                self.graph_captures += 1
+            self._graph.replay()
+            out = self._graph_out
+        self._amg_age += 1
+        return self._finish(*out)
+
+    def _launch(self, rebuild: bool):
+        """Enqueue the device work of one outer iteration; returns the Krylov infos."""
         c, fv, dim = self.case, self.fv, self.dim
         L, s = nat.lib(), nat.stream_ptr()
         M, B, R = ctypes.byref(fv.c_mesh), ctypes.byref(self.c_bc), ctypes.byref(self.c_red)
         rho, mu = c.density, c.viscosity
-        self._events = []
-        self._mark("start")
         # 1-2. boundary values with the lagged gradients, then the new gradients
         lag = self._has_lag
         nat.check(L.ftmi_simplec_bvals(M, B, nat.ptr(self.U), nat.ptr(self.p), nat.ptr(self.gU) if lag else None,
@@ -276,9 +317,13 @@ class SimpleSolver:
                                    nat.ptr(self.dP), nat.ptr(self.Fst), nat.ptr(self.cf), nat.ptr(self.Fbst),
                                    nat.ptr(self.cb), nat.ptr(self.diag_p), nat.ptr(self.offp), nat.ptr(self.mass),
                                    nat.ptr(self.rhs_p), nat.ptr(self.scal[_S_CONT:]), R, s), "rhie_chow")
+        if self._pin is not None:  # reference cells of a batch of closed cases (set_reference_cells)
+            pin, pin_he = self._pin
+            self.rhs_p.index_fill_(0, pin, 0.0)
+            self.offp.index_fill_(0, pin_he, 0.0)
         self._mark("assembly")
         # 6. pressure correction
-        if self._amg is None or self._amg_age >= self.amg_rebuild:
+        if rebuild and (self._amg is None or self._amg_age >= self.amg_rebuild):
             t0 = time.perf_counter()
             self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
             self._amg32 = self._amg.to_fp32() if self.mixed else None
@@ -287,7 +332,6 @@ class SimpleSolver:
             for k, v in self._amg.timing.items():
                 self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
             self.amg_rebuilds += 1
-        self._amg_age += 1
         if self.mixed:
             to_f32(self.offp, self.offp32)
             self.pp.zero_()
@@ -307,6 +351,12 @@ class SimpleSolver:
         if self.energy:
             tinfo = self._energy_step(L, s, M, B, R)
         self._mark("correct+energy")
+        return infos, pinfo, tinfo
+
+    def _finish(self, infos, pinfo, tinfo) -> float:
+        """Host part of an outer iteration: one read of the reduced scalars, the residual
+        normalisation, the convergence test and the stability guard."""
+        c, dim = self.case, self.dim
         sc = self.scal.cpu().numpy()  # the one host read of the iteration
         self._accumulate_timers()
         res = {}
@@ -337,6 +387,7 @@ class SimpleSolver:
         self._last_res = res
         rolled_back = self._stability_guard(self._last_raw["continuity"], flux_scale)
         if rolled_back:
+            self._graph = None  # relaxation and fields changed under the recorded graph
             self._last_res, self._last_raw = prev
             return 1.0
         if self.iteration > 5 and max(res.values()) < c.tol:
@@ -367,8 +418,27 @@ class SimpleSolver:
         else:
             info = bicgstab_ldu(self.fv, self.diagT_r, self.off, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
                                 ws=self.ws)
-        self.T, self.Tnew = self.Tnew, self.T
+        self.T.copy_(self.Tnew)  # a copy, not a swap: recorded graphs keep their pointers
         return info
+
+    def set_reference_cells(self, cells):
+        """Fix the pressure correction to zero at ``cells`` (one per closed case of a batch): their
+        rows keep only the diagonal and a zero right-hand side, and the couplings of their
+        neighbours to them are dropped. A closed case's pressure system is singular with the
+        constant as null vector; the reference removes it by projecting out the mean
+        (``simple.py:629-634``), which handles one closed case but not several side by side. With
+        a reference cell per case the system is regular, and the flux and velocity corrections,
+        which use only differences of p', are the same."""
+        dev = self.device
+        pin = torch.as_tensor(np.asarray(cells, np.int64), device=dev)
+        is_pin = np.zeros(self.nc, bool)
+        is_pin[np.asarray(cells)] = True
+        fv = self.fv
+        rows = np.repeat(np.arange(self.nc), np.diff(fv.cptr_h))
+        he = np.flatnonzero(is_pin[rows] | is_pin[fv.cnbr_h])
+        self._pin = (pin, torch.as_tensor(he, device=dev))
+        self.singular_p = False
+        self._graph = None
 
     def _fp32_outputs(self, a, nfields):
         """Point a transport assembly at the FP32 system buffers (mixed precision)."""

@@ -85,6 +85,14 @@ int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indice
 
 /* new: opt-in timing of the BiCGSTAB p-update + SpMV kernel (HIP events around each launch on
  * its stream); enable resets the totals, read waits for the recorded events */
+/* new: capture mode (1) for recording solves in a HIP graph: the Krylov solvers enqueue every
+ * iteration up to maxiter without reading the device state back (a finished solve's kernels
+ * return at once on the device), report *iters = -1 and *res = NaN, and the mixed pressure
+ * solve applies exactly one FP32 correction. Returns the previous mode. */
+int ftmi_capture_mode(int on);
+/* new: workgroup cap of the streaming kernels (default 2048 = 8 per CU; 64..65536), the knob
+ * the autotuner sets; returns the previous cap */
+int ftmi_set_grid_cap(int cap);
 int ftmi_timer_enable(int on);
 int ftmi_timer_read(double* total_ms, int64_t* launches);
 
