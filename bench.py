@@ -102,6 +102,72 @@ def calibration_copy(nbytes=256 << 20):
     return nbytes
 
 
+def batch_leg(count, steps, warmup, seed=0):
+    """256 lid-driven cavities (cells log-uniform in [16k, 200k], quads/triangles 50/50,
+    Re uniform in [100, 1000], seeded) in one concatenated mesh, mixed precision, outer
+    iterations replayed from HIP graphs, streaming-kernel grid cap autotuned."""
+    import torch
+
+    from gputest_amd.batch import BatchSolver, autotune_grid_cap, sample_cavities
+
+    specs = sample_cavities(count, seed=seed)
+    print(f"[bench:batch] building {count} cases", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    b = BatchSolver(specs, viscosity=0.01, precision="mixed", use_graph=True, max_iter=10 ** 9, tol=0.0)
+    setup = time.perf_counter() - t0
+    print(f"[bench:batch] {count} cases, {b.n_cells} cells, set-up {setup:.1f} s", file=sys.stderr, flush=True)
+    cap = autotune_grid_cap(b.solver, cache=None)
+    print(f"[bench:batch] grid cap {cap}", file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        b.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"cases": count, "cells": b.n_cells, "steps": steps, "it_per_s": round(steps / dt, 3),
+            "ms_per_step": round(dt / steps * 1e3, 2), "cell_updates_per_s": round(b.n_cells * steps / dt, 1),
+            "grid_cap": cap, "graph_recordings": b.solver.graph_captures, "amg_rebuilds": b.solver.amg_rebuilds,
+            "setup_s": round(setup, 1), "precision": "mixed", "residuals": b.solver.residuals()["scaled"]}
+
+
+def adjoint_leg(n_body, tol, max_iter, gmres_tol):
+    """Square cylinder D = 1 in a 20D x 10D channel (n_body cells per side, graded, jittered),
+    Re = 40, uniform inlet given by 64 control values: FP64 forward to convergence, then the
+    discrete-adjoint gradient of the drag with respect to the 64 values and nu."""
+    import numpy as np
+    import torch
+
+    from gputest_amd import case as C
+    from gputest_amd.adjoint import InletProfile, drag_gradient
+    from gputest_amd.mesh import square_cylinder_mesh
+    from gputest_amd.simple import SimpleSolver
+
+    mesh = square_cylinder_mesh(n_body)
+    prof = InletProfile(-5.0, 5.0, np.ones(64))
+    B = C.BoundarySpec
+    case = C.UnstructuredCase(boundaries={"inlet": B("inlet", profile=prof), "outlet": B("outlet"),
+                                          "bottom": B("symmetry"), "top": B("symmetry"), "cylinder": B("wall")},
+                              viscosity=1.0 / 40.0, max_iter=max_iter, tol=tol)
+    s = SimpleSolver(mesh, case, "cuda")
+    t0 = time.perf_counter()
+    while not s.finished:
+        s.step()
+        if s.iteration % 5000 == 0:
+            print(f"[bench:adjoint] forward iteration {s.iteration}: {s.residuals()['scaled']}", file=sys.stderr,
+                  flush=True)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t0
+    print(f"[bench:adjoint] {mesh.n_cells} cells, forward {s.iteration} iterations in {tf:.1f} s, "
+          f"residuals {s.residuals()['scaled']}", file=sys.stderr, flush=True)
+    gu, gnu, J, info = drag_gradient(s, prof, tol=gmres_tol, restart=300, maxiter=3000)
+    return {"cells": mesh.n_cells, "grads_per_s": round(1.0 / info.seconds, 5), "adjoint_s": round(info.seconds, 2),
+            "gmres_iterations": info.gmres_iterations, "gmres_residual": info.residual,
+            "forward_iterations": s.iteration, "forward_s": round(tf, 2), "forward_converged": bool(s.converged),
+            "Cd": round(2.0 * J, 6), "dCd_dnu": round(2.0 * float(gnu), 6), "parameters": 65}
+
+
 def cpu_baseline(nx, ny, iters):
     """The oracle (CPU FP64 NumPy/SciPy restatement) on the same workload: set-up excluded,
     one warm-up iteration (builds the first AMG hierarchy), then ``iters`` timed iterations."""
@@ -175,6 +241,10 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision solver (1) or not (0)")
+    ap.add_argument("--batch", type=int, default=0, help="also time a batch of this many cavities (256: the "
+                    "log-uniform 16k-200k-cell batch)")
+    ap.add_argument("--adjoint", type=int, default=0, help="also time the square-cylinder adjoint with this many "
+                    "cells per body side (100: about 0.47M triangles)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SIMPLEC path on a single MI355X "
@@ -206,6 +276,10 @@ def main():
            "roofline": roof}
     if mixed is not None:
         out["mixed"] = mixed
+    if args.batch:
+        out["batch"] = batch_leg(args.batch, steps=10, warmup=3)
+    if args.adjoint:
+        out["adjoint"] = adjoint_leg(args.adjoint, tol=1e-9, max_iter=40000, gmres_tol=1e-8)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_iters)

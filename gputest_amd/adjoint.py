@@ -150,6 +150,10 @@ class ResidualMap:
             self.inlet_idx = torch.as_tensor(self.inlet_faces, device=dev)
         self.rho = float(c.density)
         self.relax_u, self.relax_p = float(solver.relax_u), float(solver.relax_p)
+        # relaxation of the velocity update inside Phi: any value gives the same fixed point (the
+        # converged fields depend on relax_u only through the d coefficient of the flux
+        # interpolation, which keeps relax_u), and it sets how fast the adjoint GMRES converges
+        self.update_relax_u = self.relax_u
         self.simplec = solver.algo == "SIMPLEC"
         self.second_order = c.scheme == "second-order"
         self.gg = c.pressure_gradient == "green-gauss"
@@ -265,8 +269,9 @@ class ResidualMap:
         aPN, aNP = -(D + relu(-F)), -(D + relu(F))
         Db = mu * self.alpha_b * self.m_dirvel
         diag = self._sc(D + relu(F), D + relu(-F), Db + relu(Fb))
-        au = self.relax_u
+        au, au_upd = self.relax_u, self.update_relax_u
         diag_r = diag / au
+        diag_u = diag / au_upd
         Ustar = []
         for k in range(dim):
             phi, g, phib = U[:, k], gU[k], ub[:, k]
@@ -283,8 +288,8 @@ class ResidualMap:
                 dc = F * (ho - torch.where(up, phP, phN))
                 rhs = rhs - self._sc(dc, -dc)
             rhs = rhs + src[:, k]
-            b = rhs + ((1.0 - au) / au) * diag * phi
-            Ustar.append(_MomentumSolve.apply(diag_r, aPN, aNP, b, phi.detach(), self))
+            b = rhs + ((1.0 - au_upd) / au_upd) * diag * phi
+            Ustar.append(_MomentumSolve.apply(diag_u, aPN, aNP, b, phi.detach(), self))
         Ustar = torch.stack(Ustar, dim=1)
         if self.simplec:
             denom = diag_r - self._sc(-aPN, -aNP)
