@@ -132,7 +132,8 @@ class SmoothedAggregationAMG:
         cnt = torch.zeros(n, dtype=torch.int32, device=dev)  # rows over the hash capacity stay 0
         long_rows = torch.empty(n, dtype=torch.int32, device=dev)
         n_long = torch.zeros(1, dtype=torch.int32, device=dev)
-        args = (n, nat.ptr(X.ptr), nat.ptr(X.col), nat.ptr(X.val), nat.ptr(Y.ptr), nat.ptr(Y.col), nat.ptr(Y.val),
+        args = (n, Y.ncols, nat.ptr(X.ptr), nat.ptr(X.col), nat.ptr(X.val), nat.ptr(Y.ptr), nat.ptr(Y.col),
+                nat.ptr(Y.val),
                 nat.ptr(xscale), nat.ptr(agg), nat.ptr(tval), nat.ptr(omega_num), float(omega_scale))
         work = (nat.ptr(self._ovf), nat.ptr(long_rows), nat.ptr(n_long), st)
         nat.check(L.ftmi_amgs_spgemm(0, *args, nat.ptr(cnt), None, None, None, *work), "spgemm")

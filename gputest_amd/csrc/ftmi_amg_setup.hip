@@ -10,7 +10,8 @@
 // Sparse products run one thread per output row as a k-way merge of the right operand's rows
 // (rows of the left operand up to 32 entries); longer rows go to one wavefront each, whose
 // lanes insert the row's candidate columns into an LDS hash table (integer compare-and-swap,
-// so the set is deterministic) and rank the distinct columns into sorted order. Either way
+// so the set is deterministic), accumulate staged products in LDS and sort the distinct
+// columns with a bitonic network. Either way
 // each column's value is summed in the fixed order of the left operand's row, so results are
 // independent of the schedule and equal a row-by-row sparse product. Rows are sized in a
 // count pass, the caller scans the counts into row pointers and allocates, and a fill pass
@@ -191,6 +192,27 @@ __device__ inline int32_t find_in_row(const CsrV& Y, int32_t k, int32_t c) {
 // below), one wavefront per row, Xs = diag(xscale) X when xscale != nullptr. In "prolongator"
 // mode (agg != nullptr) the row is finished as C_rc = [c == agg[r]] t[r] - omega C_rc, the
 // smoothed prolongator T - omega D^-1 A T with Y = T.
+//
+// The lanes insert the row's candidate columns into an LDS hash table (integer CAS, so the set
+// of distinct columns does not depend on the schedule). In the fill pass the candidates'
+// products (xs x_rj) y_jc are staged in LDS with the hash slot of their column, a run of
+// consecutive j at a time (as many as the staging arrays hold), and added into per-slot
+// accumulators one j at a time in the order of X's row (the columns of one Y row are distinct,
+// so the lanes of one step never collide): the order a row-by-row sparse product sums in. The
+// distinct columns are sorted by a bitonic network and written out with their accumulated
+// values. Rows longer than kMaxX, or with a Y row longer than kCand, fall back to summing each
+// column over X's row with a binary search in the Y rows (same order, slower).
+constexpr int kCand = 2048;  // staged candidates per run
+constexpr int kMaxX = 1024;  // longest X row that is staged
+
+// slot of column c in the hash table (c is present)
+__device__ inline int find_slot(const int32_t* keys, int32_t c, int bits) {
+    uint32_t h = hslot(c, bits);
+    const int mask = (1 << bits) - 1;
+    while (keys[h] != c) h = (h + 1) & mask;
+    return (int)h;
+}
+
 template <bool FILL>
 __global__ void __launch_bounds__(kWave) k_spgemm(const int32_t* __restrict__ rows, const int32_t* n_rows, CsrV X,
                                                   CsrV Y, const double* __restrict__ xscale,
@@ -201,25 +223,58 @@ __global__ void __launch_bounds__(kWave) k_spgemm(const int32_t* __restrict__ ro
                                                   int* overflow) {
     __shared__ int32_t keys[kHash];
     __shared__ int32_t dense[kHash];
+    __shared__ double acc[FILL ? kHash : 1];
+    __shared__ double stv[FILL ? kCand : 1];
+    __shared__ int16_t sts[FILL ? kCand : 1];
+    __shared__ int32_t jptr[FILL ? kMaxX + 1 : 1];
     __shared__ int32_t nk, nd;
     const int lane = threadIdx.x;
     const int64_t nlist = *n_rows;
     for (int64_t li = blockIdx.x; li < nlist; li += gridDim.x) {
         const int64_t r = rows[li];
         const int32_t xb = X.ptr[r], xe = X.ptr[r + 1];
+        const int32_t L = xe - xb;
+        // candidate count, and (fill pass) the offsets of each j's candidates
+        int bound = 0, longest = 0;
+        const bool stage_ok = FILL && L <= kMaxX;
+        if (stage_ok && lane == 0) jptr[0] = 0;
+        for (int32_t j0 = 0; j0 < L; j0 += kWave) {
+            const int32_t j = j0 + lane;
+            int len = 0;
+            if (j < L) {
+                const int32_t k = X.col[xb + j];
+                len = Y.ptr[k + 1] - Y.ptr[k];
+            }
+            longest = len > longest ? len : longest;
+            int inc = len;  // inclusive scan over the wave
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int t = __shfl_up(inc, off, kWave);
+                if (lane >= off) inc += t;
+            }
+            if (stage_ok && j < L) jptr[j + 1] = bound + inc;
+            bound += __shfl(inc, kWave - 1, kWave);
+        }
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) {
+            const int t = __shfl_xor(longest, off, kWave);
+            longest = t > longest ? t : longest;
+        }
+        const bool staged = stage_ok && longest <= kCand;
         // table size: a power of two >= twice this row's candidate count (at least 64 slots)
-        int bound = 0;
-        for (int32_t j = xb + lane; j < xe; j += kWave) bound += Y.ptr[X.col[j] + 1] - Y.ptr[X.col[j]];
-        bound = wave_sum(bound);
         int bits = 6;
         while (bits < 11 && (1 << bits) < 2 * bound) ++bits;
         const int hsize = 1 << bits;
-        for (int h = lane; h < hsize; h += kWave) keys[h] = -1;
+        for (int h = lane; h < hsize; h += kWave) {
+            keys[h] = -1;
+            if (FILL) acc[h] = 0.0;
+        }
         if (lane == 0) {
             nk = 0;
             nd = 0;
         }
         __syncthreads();
+        const double xs = (FILL && xscale) ? xscale[r] : 1.0;
         for (int32_t j = xb + lane; j < xe; j += kWave) {
             const int32_t k = X.col[j];
             for (int32_t q = Y.ptr[k]; q < Y.ptr[k + 1]; ++q) {
@@ -255,35 +310,153 @@ __global__ void __launch_bounds__(kWave) k_spgemm(const int32_t* __restrict__ ro
             __syncthreads();
             continue;
         }
-        // compact the distinct columns (any order), then rank them: keys[rank(c)] = c
+        // the distinct columns, compacted (any order) and sorted ascending
         for (int h = lane; h < hsize; h += kWave) {
             const int32_t c = keys[h];
             if (c >= 0) dense[atomicAdd(&nd, 1)] = c;
         }
+        int m = kWave;
+        while (m < n_keys) m <<= 1;  // <= kHash, as n_keys < hsize <= kHash
         __syncthreads();
+        for (int t = n_keys + lane; t < m; t += kWave) dense[t] = 0x7fffffff;
+        __syncthreads();
+        for (int k = 2; k <= m; k <<= 1) {
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int i = lane; i < m; i += kWave) {
+                    const int ixj = i ^ jj;
+                    if (ixj > i) {
+                        const int32_t a = dense[i], b = dense[ixj];
+                        if ((a > b) == ((i & k) == 0)) {
+                            dense[i] = b;
+                            dense[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (staged) {
+            for (int32_t jb = 0; jb < L;) {
+                // the run [jb, je): as many j as fit the staging arrays (at least one)
+                int32_t je = jb + 1;
+                while (je < L && jptr[je + 1] - jptr[jb] <= kCand) ++je;
+                const int32_t p0 = jptr[jb];
+                for (int32_t j = jb + lane; j < je; j += kWave) {
+                    const int32_t k = X.col[xb + j];
+                    const int32_t qb = Y.ptr[k], qe = Y.ptr[k + 1];
+                    const double xv = xscale ? xs * X.val[xb + j] : X.val[xb + j];
+                    const int32_t base = jptr[j] - p0 - qb;
+                    for (int32_t q = qb; q < qe; ++q) {
+                        sts[base + q] = (int16_t)find_slot(keys, Y.col[q], bits);
+                        stv[base + q] = xv * Y.val[q];
+                    }
+                }
+                __syncthreads();
+                // one j at a time, in X's order
+                for (int32_t j = jb; j < je; ++j) {
+                    const int32_t pb = jptr[j] - p0, pe = jptr[j + 1] - p0;
+                    for (int32_t p = pb + lane; p < pe; p += kWave) acc[sts[p]] += stv[p];
+                    __syncthreads();
+                }
+                jb = je;
+            }
+        }
+        const int32_t base = cptr[r];
+        const double om = agg ? omega_scale / *omega_num : 0.0;  // (4/3) / rho
         for (int t = lane; t < n_keys; t += kWave) {
             const int32_t c = dense[t];
-            int rank = 0;
-            for (int u = 0; u < n_keys; ++u) rank += dense[u] < c ? 1 : 0;
-            keys[rank] = c;
-        }
-        __syncthreads();
-        const int32_t base = cptr[r];
-        const double xs = xscale ? xscale[r] : 1.0;
-        for (int t = lane; t < n_keys; t += kWave) {
-            const int32_t c = keys[t];
             double v = 0.0;
-            for (int32_t j = xb; j < xe; ++j) {
-                const int32_t q = find_in_row(Y, X.col[j], c);
-                if (q >= 0) v += (xscale ? xs * X.val[j] : X.val[j]) * Y.val[q];
+            if (staged) {
+                v = acc[find_slot(keys, c, bits)];
+            } else {
+                for (int32_t j = xb; j < xe; ++j) {
+                    const int32_t q = find_in_row(Y, X.col[j], c);
+                    if (q >= 0) v += (xscale ? xs * X.val[j] : X.val[j]) * Y.val[q];
+                }
             }
-            if (agg) {
-                const double om = omega_scale / *omega_num;  // (4/3) / rho
-                v = (c == agg[r] ? tval[r] : 0.0) - om * v;
-            }
+            if (agg) v = (c == agg[r] ? tval[r] : 0.0) - om * v;
             ccol[base + t] = c;
             cval[base + t] = v;
         }
+        __syncthreads();
+    }
+}
+
+// C = Xs Y one wavefront per row when C has few columns (the coarse levels): a dense LDS
+// accumulator indexed by column, with one flag bit per column, replaces the hash table and the
+// sort. The products are added one j at a time in the order of X's row (the columns of one Y
+// row are distinct, so the lanes of one step never collide). The touched columns are written
+// out in column order (a wave prefix scan of the flag words' popcounts, then each lane's bits
+// in ascending order), and the accumulators and flags they used are cleared on the way, so a
+// row costs its candidates plus ncols / 2048 wave steps.
+constexpr int kDenseCols = 4096;
+constexpr int64_t kDenseWork = int64_t(1) << 24;  // rows x columns up to which the dense kernel is used
+
+template <bool FILL>
+__global__ void __launch_bounds__(kWave) k_spgemm_dense(int64_t nrows, int32_t ncols, CsrV X, CsrV Y,
+                                                        const double* __restrict__ xscale,
+                                                        const int32_t* __restrict__ agg,
+                                                        const double* __restrict__ tval, const double* omega_num,
+                                                        double omega_scale, int32_t* __restrict__ rowcnt,
+                                                        const int32_t* __restrict__ cptr,
+                                                        int32_t* __restrict__ ccol, double* __restrict__ cval) {
+    extern __shared__ double dense_lds[];  // FILL: ncols accumulators; then the flag words
+    double* acc = dense_lds;
+    uint32_t* flag = (uint32_t*)(dense_lds + (FILL ? ncols : 0));
+    const int32_t nw = (ncols + 31) >> 5;
+    const int lane = threadIdx.x;
+    const double om = (FILL && agg) ? omega_scale / *omega_num : 0.0;  // (4/3) / rho
+    if (FILL)
+        for (int32_t c = lane; c < ncols; c += kWave) acc[c] = 0.0;
+    for (int32_t w = lane; w < nw; w += kWave) flag[w] = 0u;
+    __syncthreads();
+    for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+        const int32_t xb = X.ptr[r], xe = X.ptr[r + 1];
+        const double xs = (FILL && xscale) ? xscale[r] : 1.0;
+        for (int32_t j = xb; j < xe; ++j) {
+            const int32_t k = X.col[j];
+            const double xv = FILL ? (xscale ? xs * X.val[j] : X.val[j]) : 0.0;
+            for (int32_t q = Y.ptr[k] + lane; q < Y.ptr[k + 1]; q += kWave) {
+                const int32_t c = Y.col[q];
+                atomicOr(&flag[c >> 5], 1u << (c & 31));
+                if (FILL) acc[c] += xv * Y.val[q];
+            }
+            if (FILL) __syncthreads();
+        }
+        __syncthreads();
+        int32_t cnt = 0;
+        const int32_t base = FILL ? cptr[r] : 0;
+        const int32_t ar = (FILL && agg) ? agg[r] : -1;
+        const double tr = (FILL && agg) ? tval[r] : 0.0;
+        for (int32_t w0 = 0; w0 < nw; w0 += kWave) {
+            const int32_t w = w0 + lane;
+            uint32_t word = w < nw ? flag[w] : 0u;
+            const int pc = __popc(word);
+            int inc = pc;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const int t = __shfl_up(inc, off, kWave);
+                if (lane >= off) inc += t;
+            }
+            if (word) {
+                flag[w] = 0u;
+                int32_t pos = cnt + inc - pc;
+                while (word) {
+                    const int32_t c = (w << 5) + (__ffs(word) - 1);
+                    word &= word - 1u;
+                    if (FILL) {
+                        double v = acc[c];
+                        acc[c] = 0.0;
+                        if (agg) v = (c == ar ? tr : 0.0) - om * v;
+                        ccol[base + pos] = c;
+                        cval[base + pos] = v;
+                    }
+                    ++pos;
+                }
+            }
+            cnt += __shfl(inc, kWave - 1, kWave);
+        }
+        if (!FILL && lane == 0) rowcnt[r] = cnt;
         __syncthreads();
     }
 }
@@ -292,9 +465,11 @@ __global__ void __launch_bounds__(kWave) k_spgemm(const int32_t* __restrict__ ro
 // the (column-sorted) rows of Y selected by the row of X. Each step takes the smallest head
 // column, adds the contributions of the heads at that column in the order of X's row (the
 // order a row-by-row sparse product accumulates in) and advances them, so the row comes out
-// sorted with no hash and no LDS. Longer rows are appended to long_rows in the count pass and
-// handled by k_spgemm.
+// sorted with no hash and no LDS. Longer rows, and rows with more than kMergeWork candidates
+// (a thread would step through hundreds of output columns alone), are appended to long_rows in
+// the count pass and handled by k_spgemm.
 constexpr int kMergeCap = 32;
+constexpr int kMergeWork = 1024;
 
 template <bool FILL>
 __global__ void __launch_bounds__(kBlock) k_spgemm_merge(int64_t nrows, CsrV X, CsrV Y,
@@ -317,6 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_spgemm_merge(int64_t nrows, CsrV X, 
         }
         int32_t q[kMergeCap], e[kMergeCap], c[kMergeCap];
         int32_t cur = kNone;
+        int32_t work = 0;
 #pragma unroll
         for (int t = 0; t < kMergeCap; ++t) {
             q[t] = 0;
@@ -326,9 +502,14 @@ __global__ void __launch_bounds__(kBlock) k_spgemm_merge(int64_t nrows, CsrV X, 
                 const int32_t k = X.col[xb + t];
                 q[t] = Y.ptr[k];
                 e[t] = Y.ptr[k + 1];
+                work += e[t] - q[t];
                 if (q[t] < e[t]) c[t] = Y.col[q[t]];
             }
             cur = c[t] < cur ? c[t] : cur;
+        }
+        if (work > kMergeWork) {
+            if (!FILL) long_rows[atomicAdd(n_long, 1)] = (int32_t)r;
+            continue;
         }
         const double xs = (FILL && xscale) ? xscale[r] : 1.0;
         const int32_t ar = (FILL && agg) ? agg[r] : -1;
@@ -548,10 +729,13 @@ int ftmi_amgs_power(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* c
     return 0;
 }
 
-// Row counts (fill == 0) or entries (fill == 1) of C = diag(xscale) X Y; with agg != NULL,
-// C = T - (omega_scale / *omega_num) diag(xscale) X Y with T = (agg, tval). overflow (4 int)
-// records rows whose distinct columns exceed the LDS hash capacity (count, max, bound, row).
-int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_t* x_col, const double* x_val,
+// Row counts (fill == 0) or entries (fill == 1) of C = diag(xscale) X Y (C has ncols columns);
+// with agg != NULL, C = T - (omega_scale / *omega_num) diag(xscale) X Y with T = (agg, tval).
+// overflow (4 int) records rows whose distinct columns exceed the LDS hash capacity (count,
+// max, bound, row). Small products (kDenseCols columns, kDenseWork rows x columns) use the
+// dense-accumulator kernel.
+int ftmi_amgs_spgemm(int fill, int64_t nrows, int64_t ncols, const int32_t* x_ptr, const int32_t* x_col,
+                     const double* x_val,
                      const int32_t* y_ptr, const int32_t* y_col, const double* y_val, const double* xscale,
                      const int32_t* agg, const double* tval, const double* omega_num, double omega_scale,
                      int32_t* rowcnt, const int32_t* c_ptr, int32_t* c_col, double* c_val, int* overflow,
@@ -565,7 +749,18 @@ int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_
     CsrV X{x_ptr, x_col, x_val}, Y{y_ptr, y_col, y_val};
     const int g = grid_for(nrows);
     const int gl = nrows < kMaxGrid ? (int)nrows : kMaxGrid;
-    if (fill) {
+    if (ncols <= kDenseCols && nrows * ncols <= kDenseWork) {
+        const int gd = nrows < 2 * kMaxGrid ? (int)nrows : 2 * kMaxGrid;
+        const int nc = (int)ncols;
+        const size_t words = (size_t)((nc + 31) / 32) * 4;
+        if (fill)
+            k_spgemm_dense<true><<<gd, kWave, (size_t)nc * 8 + words, s>>>(nrows, nc, X, Y, xscale, agg, tval,
+                                                                         omega_num, omega_scale, rowcnt, c_ptr,
+                                                                         c_col, c_val);
+        else
+            k_spgemm_dense<false><<<gd, kWave, words, s>>>(nrows, nc, X, Y, xscale, agg, tval, omega_num,
+                                                         omega_scale, rowcnt, c_ptr, c_col, c_val);
+    } else if (fill) {
         k_spgemm_merge<true><<<g, kBlock, 0, s>>>(nrows, X, Y, xscale, agg, tval, omega_num, omega_scale, rowcnt,
                                                   c_ptr, c_col, c_val, long_rows, n_long);
         k_spgemm<true><<<gl, kWave, 0, s>>>(long_rows, n_long, X, Y, xscale, agg, tval, omega_num, omega_scale,

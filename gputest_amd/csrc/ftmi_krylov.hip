@@ -11,8 +11,9 @@
 // is computed by the last workgroup of the reducing kernel, so an iteration is a fixed
 // sequence of launches with no host round trip. Each kernel first reads the device `done`
 // flag (set only by an earlier kernel) and returns at once when the solve has finished.
-// The host checks `done` one iteration behind, through an asynchronous copy into pinned
-// memory, so the GPU never waits for the host.
+// The host checks `done` one iteration behind: the finalize kernel of each iteration writes the
+// state block into coherent pinned host memory and stamps it with a sequence number, which the
+// host polls, so the GPU never waits for the host and no copy or event is queued per iteration.
 //
 // The BiCGSTAB iterate lives in two buffers (the caller's x and a work buffer): the update
 // x + alpha phat + omega shat is written to the buffer that is not current and becomes
@@ -39,10 +40,11 @@ namespace {
 enum { S_RHO, S_ALPHA, S_OMEGA, S_RNORM, S_BNORM, S_BABS, S_RHONEW, S_RES, S_RZ, S_RZ1, S_ZMEAN, S_BMEAN, S_NS = 16 };
 enum { I_DONE, I_CONV, I_ITERS, I_CUR, I_FIXUP, I_NI = 8 };
 
-struct State {  // 192 bytes, copied to the host as one block
+struct State {  // 192 bytes; `seq` stamps a host copy (see post_host)
     double sc[S_NS];
     int st[I_NI];
-    int pad[8];
+    unsigned seq;
+    int pad[7];
 };
 
 __global__ void k_state_reset(State* s, unsigned* counter) {
@@ -53,14 +55,19 @@ __global__ void k_state_reset(State* s, unsigned* counter) {
 }
 
 // ------------------------------------------------------------------ host-side flag reader
+// Two host slots in coherent (fine-grained) pinned memory: finalize kernels store the state
+// block there directly and stamp it, the host polls the stamp.
 struct Pinned {
-    State* h = nullptr;  // two slots
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    State* h = nullptr;  // host view
+    State* d = nullptr;  // device view of the same memory
+    unsigned want[2] = {0u, 0u};
+    unsigned seq = 0u;
     bool ok = false;
     Pinned() {
-        if (hipHostMalloc((void**)&h, 2 * sizeof(State), hipHostMallocDefault) != hipSuccess) return;
-        if (hipEventCreateWithFlags(&ev[0], hipEventDisableTiming) != hipSuccess) return;
-        if (hipEventCreateWithFlags(&ev[1], hipEventDisableTiming) != hipSuccess) return;
+        if (hipHostMalloc((void**)&h, 2 * sizeof(State), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess)
+            return;
+        std::memset(h, 0, 2 * sizeof(State));
+        if (hipHostGetDevicePointer((void**)&d, h, 0) != hipSuccess) return;
         ok = true;
     }
 };
@@ -70,12 +77,60 @@ static Pinned* pinned() {
     return g_pinned->ok ? g_pinned : nullptr;
 }
 
-static int post_state(Pinned* pn, int slot, const State* d, hipStream_t s) {
-    if (hipMemcpyAsync(&pn->h[slot], d, sizeof(State), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
-    if (hipEventRecord(pn->ev[slot], s) != hipSuccess) return -1;
+// Where the next finalize kernel posts: the device address of `slot` and a fresh stamp.
+struct Post {
+    State* m;
+    unsigned seq;
+};
+static Post next_post(Pinned* pn, int slot) {
+    pn->want[slot] = ++pn->seq;
+    if (pn->seq == 0u) pn->want[slot] = ++pn->seq;  // 0 is the cleared value
+    return Post{pn->d + slot, pn->want[slot]};
+}
+static constexpr Post kNoPost{nullptr, 0u};
+
+// Thread 0 of a finalize kernel: copy the state block to the host slot, then (after a
+// system-scope fence) its stamp, so a host that sees the stamp sees the fields.
+__device__ inline void post_host(const State* S, Post P) {
+    if (!P.m || threadIdx.x != 0) return;
+    double sc[S_NS];  // all loads first, then all stores: one wait instead of one per field
+    int st[I_NI];
+#pragma unroll
+    for (int j = 0; j < S_NS; ++j) sc[j] = S->sc[j];
+#pragma unroll
+    for (int j = 0; j < I_NI; ++j) st[j] = S->st[j];
+#pragma unroll
+    for (int j = 0; j < S_NS; ++j) P.m->sc[j] = sc[j];
+#pragma unroll
+    for (int j = 0; j < I_NI; ++j) P.m->st[j] = st[j];
+    __threadfence_system();
+    __hip_atomic_store(&P.m->seq, P.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Spin until slot `slot` carries its stamp. The stream is queried now and then, so a failed
+// launch is reported and a drained stream without the stamp is an error instead of a hang.
+static int wait_state(Pinned* pn, int slot, hipStream_t st) {
+    const unsigned want = pn->want[slot];
+    const volatile unsigned* p = &pn->h[slot].seq;
+    for (unsigned spin = 1u;; ++spin) {
+        if (*p == want) break;
+        if ((spin & 255u) == 0u) {
+            const hipError_t e = hipStreamQuery(st);
+            if (e == hipSuccess) {
+                if (*p == want) break;
+                set_error_msg("krylov: the solver state was not posted");
+                return -1;
+            }
+            if (e != hipErrorNotReady) {
+                set_error_msg("krylov: stream error while waiting for the solver state");
+                return -1;
+            }
+        }
+        __builtin_ia32_pause();
+    }
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
     return 0;
 }
-static int wait_state(Pinned* pn, int slot) { return hipEventSynchronize(pn->ev[slot]) == hipSuccess ? 0 : -1; }
 
 // ------------------------------------------------------------------ opt-in kernel timer
 // When enabled, the launches of one kernel (the BiCGSTAB p-update + SpMV, the solver's
@@ -154,7 +209,7 @@ __global__ void __launch_bounds__(kBlock) bi_init(int64_t n, Op A, const T* __re
 }
 
 __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, int nparts, State* S, double tol,
-                                                      int rel) {
+                                                      int rel, Post post) {
     double tot[2];
     if (finalize_sum<2>(partials, nparts, tot)) {
         const double rnorm = sqrt(tot[0]), b_abs = sqrt(tot[1]);
@@ -176,6 +231,7 @@ __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, in
             st[I_ITERS] = 0;
         }
     }
+    post_host(S, post);
 }
 
 // p = r + beta (p_prev - omega v_prev), phat = inv p, v = A phat; partials of rhat . v
@@ -291,10 +347,9 @@ __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const T* __restrict__ 
 }
 
 __global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int nparts, State* S, double tol, int k,
-                                                   int maxiter) {
-    if (S->st[I_DONE]) return;
+                                                   int maxiter, Post post) {
     double tot[2];
-    if (finalize_sum<2>(partials, nparts, tot)) {
+    if (!S->st[I_DONE] && finalize_sum<2>(partials, nparts, tot)) {
         double* sc = S->sc;
         int* st = S->st;
         const int cur = st[I_CUR];
@@ -305,28 +360,29 @@ __global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int n
             st[I_CONV] = 0;
             st[I_ITERS] = k;
             sc[S_RES] = __builtin_nan("");
-            return;
-        }
-        st[I_CUR] = cur ^ 1;
-        sc[S_RNORM] = rnorm;
-        sc[S_RES] = rnorm / sc[S_BNORM];
-        if (rnorm / sc[S_BNORM] < tol || rnorm <= 1e-14 * sc[S_BABS]) {
-            st[I_DONE] = 1;
-            st[I_CONV] = 1;
-            st[I_ITERS] = k;
-        } else if (om == 0.0) {
-            st[I_DONE] = 1;
-            st[I_CONV] = 0;
-            st[I_ITERS] = k;
         } else {
-            sc[S_RHONEW] = tot[1];
-            if (tot[1] == 0.0 && k < maxiter) {  // the next iteration breaks at its first step
+            st[I_CUR] = cur ^ 1;
+            sc[S_RNORM] = rnorm;
+            sc[S_RES] = rnorm / sc[S_BNORM];
+            if (rnorm / sc[S_BNORM] < tol || rnorm <= 1e-14 * sc[S_BABS]) {
+                st[I_DONE] = 1;
+                st[I_CONV] = 1;
+                st[I_ITERS] = k;
+            } else if (om == 0.0) {
                 st[I_DONE] = 1;
                 st[I_CONV] = 0;
-                st[I_ITERS] = k + 1;
+                st[I_ITERS] = k;
+            } else {
+                sc[S_RHONEW] = tot[1];
+                if (tot[1] == 0.0 && k < maxiter) {  // the next iteration breaks at its first step
+                    st[I_DONE] = 1;
+                    st[I_CONV] = 0;
+                    st[I_ITERS] = k + 1;
+                }
             }
         }
     }
+    post_host(S, post);
 }
 
 // x0 = x_cur + alpha phat (early exit on ||s||), or x0 = x1 when x1 is current.
@@ -427,7 +483,7 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
         bi_init<Op, T, true><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
     else
         bi_init<Op, T, false><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
-    fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel);
+    fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel, g_capture ? kNoPost : next_post(pn, 0));
     FTMI_LAUNCH_CHECK("bicgstab init");
     if (g_capture) {
         for (int64_t k = 1; k <= maxiter; ++k) {
@@ -438,7 +494,7 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
                                                (int)k);
             bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
                                           tol, (int)k);
-            fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter);
+            fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, kNoPost);
         }
         bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, W.S);
         FTMI_LAUNCH_CHECK("bicgstab (capture)");
@@ -446,7 +502,6 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
         *res = __builtin_nan("");
         return 0;
     }
-    if (post_state(pn, 0, W.S, st)) return -1;
     // slot `last` holds the most recently posted state; the loop waits one iteration behind
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
@@ -467,16 +522,15 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
                                            (int)k);
         bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
                                       tol, (int)k);
-        fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter);
+        fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, next_post(pn, last ^ 1));
         FTMI_LAUNCH_CHECK("bicgstab iteration");
-        if (post_state(pn, last ^ 1, W.S, st)) return -1;
-        if (wait_state(pn, last)) return -1;  // the previous iteration's flags
+        if (wait_state(pn, last, st)) return -1;  // the previous iteration's flags
         const bool prev_done = pn->h[last].st[I_DONE] != 0;
         last ^= 1;
         if (prev_done) break;
     }
     // every posted state is final once the solve is done; the last one is the latest
-    if (wait_state(pn, last)) return -1;
+    if (wait_state(pn, last, st)) return -1;
     const int slot = last;
     const State& H = pn->h[slot];
     int conv;
@@ -1019,7 +1073,8 @@ __global__ void __launch_bounds__(kBlock) pcg_init_b(int64_t n, const T* __restr
     block_partials<1>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_pcg_init(const double* partials, int nparts, State* S, double tol) {
+__global__ void __launch_bounds__(kBlock) fin_pcg_init(const double* partials, int nparts, State* S, double tol,
+                                                       Post post) {
     double tot[1];
     if (finalize_sum<1>(partials, nparts, tot)) {
         const double rn = sqrt(tot[0]);
@@ -1031,6 +1086,7 @@ __global__ void __launch_bounds__(kBlock) fin_pcg_init(const double* partials, i
             S->st[I_ITERS] = 0;
         }
     }
+    post_host(S, post);
 }
 
 // z -= mean(z) (mean from the V-cycle's partial sums); partials r.z
@@ -1132,10 +1188,10 @@ __global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const T* __restrict_
     block_partials<1>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int nparts, State* S, double tol, int k) {
-    if (S->st[I_DONE]) return;
+__global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int nparts, State* S, double tol, int k,
+                                                     Post post) {
     double tot[1];
-    if (finalize_sum<1>(partials, nparts, tot)) {
+    if (!S->st[I_DONE] && finalize_sum<1>(partials, nparts, tot)) {
         const double rn = sqrt(tot[0]);
         S->sc[S_RNORM] = rn;
         S->sc[S_RES] = rn / S->sc[S_BNORM];
@@ -1145,6 +1201,7 @@ __global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int
             S->st[I_ITERS] = k;
         }
     }
+    post_host(S, post);
 }
 
 template <typename T>
@@ -1199,7 +1256,7 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
         pcg_init_a<false, T><<<g, kBlock, 0, st>>>(n, b, P0);
         pcg_init_b<false, T, XT><<<g, kBlock, 0, st>>>(n, b, W.r, x, zx, W.S, P0, g, P1);
     }
-    fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol);
+    fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol, g_capture ? kNoPost : next_post(pn, 0));
     // z = M r, with the partial sums of z in P2 when projecting; returns their count
     auto precond = [&]() -> int {
         if (M && M->nlev > 0) {
@@ -1232,7 +1289,7 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
             const int a = (int)((k - 1) & 1), c = (int)(k & 1);
             pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
             pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
-            fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k);
+            fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, kNoPost);
             nz = precond();
             if (project)
                 pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
@@ -1244,26 +1301,24 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
         *res = __builtin_nan("");
         return 0;
     }
-    if (post_state(pn, 0, W.S, st)) return -1;
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
         pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
         pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
-        fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k);
+        fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, next_post(pn, last ^ 1));
         nz = precond();
         if (project)
             pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         else
             pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         FTMI_LAUNCH_CHECK("pcg iteration");
-        if (post_state(pn, last ^ 1, W.S, st)) return -1;
-        if (wait_state(pn, last)) return -1;
+        if (wait_state(pn, last, st)) return -1;
         const bool prev_done = pn->h[last].st[I_DONE] != 0;
         last ^= 1;
         if (prev_done) break;
     }
-    if (wait_state(pn, last)) return -1;
+    if (wait_state(pn, last, st)) return -1;
     const int slot = last;
     const State& H = pn->h[slot];
     if (H.st[I_DONE]) {
@@ -1331,7 +1386,8 @@ __global__ void __launch_bounds__(kBlock) k_project_round(int64_t n, double* __r
 }
 
 // out[0] = ||b||^2, out[1] = ||projected r||^2
-__global__ void __launch_bounds__(kBlock) fin_mixed_norms(const double* p3, const double* p1, int nparts, State* S) {
+__global__ void __launch_bounds__(kBlock) fin_mixed_norms(const double* p3, const double* p1, int nparts, State* S,
+                                                          Post post) {
     double t3[3], t1[1];
     const bool lead3 = finalize_sum<3>(p3, nparts, t3);
     const bool lead1 = finalize_sum<1>(p1, nparts, t1);
@@ -1339,6 +1395,7 @@ __global__ void __launch_bounds__(kBlock) fin_mixed_norms(const double* p3, cons
         S->sc[S_BABS] = sqrt(t3[2]);
         S->sc[S_RNORM] = sqrt(t1[0]);
     }
+    post_host(S, post);
 }
 
 // ex = (float)(d + sum_k off_k): the row sums of the FP64 operator, rounded once
@@ -1466,9 +1523,9 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
         // the FP64 residual of the current iterate, projected, and its norm (one host read)
         k_resid_f64<<<g, kBlock, 0, st>>>(n, A64, b, x, r64, pa);
         k_project_round<<<g, kBlock, 0, st>>>(n, r64, pa, g, project, r32, pb);
-        fin_mixed_norms<<<1, kBlock, 0, st>>>(pa, pb, g, S);
+        fin_mixed_norms<<<1, kBlock, 0, st>>>(pa, pb, g, S, next_post(pn, 0));
         FTMI_LAUNCH_CHECK("ftmi_pcg_ldu_mixed residual");
-        if (post_state(pn, 0, S, st) || wait_state(pn, 0)) return -1;
+        if (wait_state(pn, 0, st)) return -1;
         const double babs = pn->h[0].sc[S_BABS], rnorm = pn->h[0].sc[S_RNORM];
         if (round == 0) bnorm = babs != 0.0 ? babs : 1.0;
         rel = rnorm / bnorm;

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 100
+#define FTMI_ABI_VERSION 101
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -193,12 +193,14 @@ int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, 
  * 4096 doubles; out[0] = the estimate (device) */
 int ftmi_amgs_power(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
                     const double* dinv, double* x, double* y, int iters, double* partials, double* out, void* stream);
-/* deterministic sparse product C = diag(xscale) X Y, count (fill = 0) or fill pass; with agg:
- * C = T - (omega_scale / *omega_num) diag(xscale) X Y (the smoothed prolongator). Columns come
- * out sorted and each entry is summed in the order of X's row. long_rows (nrows int32) and
- * n_long (1 int32, zero before the count pass) are work shared by the two passes: the count
- * pass lists the rows of X longer than 32 entries there, and both passes read that list. */
-int ftmi_amgs_spgemm(int fill, int64_t nrows, const int32_t* x_ptr, const int32_t* x_col, const double* x_val,
+/* deterministic sparse product C = diag(xscale) X Y (nrows x ncols), count (fill = 0) or fill
+ * pass; with agg: C = T - (omega_scale / *omega_num) diag(xscale) X Y (the smoothed
+ * prolongator). Columns come out sorted and each entry is summed in the order of X's row.
+ * long_rows (nrows int32) and n_long (1 int32, zero before the count pass) are work shared by
+ * the two passes: the count pass lists the rows of X too long for one thread there, and both
+ * passes read that list. */
+int ftmi_amgs_spgemm(int fill, int64_t nrows, int64_t ncols, const int32_t* x_ptr, const int32_t* x_col,
+                     const double* x_val,
                      const int32_t* y_ptr, const int32_t* y_col, const double* y_val, const double* xscale,
                      const int32_t* agg, const double* tval, const double* omega_num, double omega_scale,
                      int32_t* rowcnt, const int32_t* c_ptr, int32_t* c_col, double* c_val, int* overflow,
