@@ -17,8 +17,8 @@ from __future__ import annotations
 import json
 import os
 import time
-from dataclasses import dataclass
-from typing import Dict, List, Optional, Sequence
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence
 
 import numpy as np
 import torch
@@ -150,6 +150,136 @@ class BatchSolver:
         return out
 
 
+# ---------------------------------------------------------------- job-list driver
+@dataclass
+class BatchJob:
+    """One case of a batch (``runtime/batch.py:34-40``): a Fluent mesh file, the
+    ``UnstructuredCase`` keyword arguments (``boundaries`` as dicts of ``BoundarySpec`` fields,
+    overriding ``default_boundaries`` of the mesh), and an optional output folder."""
+    name: str
+    mesh: str
+    case: dict = field(default_factory=dict)
+    out_dir: str = ""
+    solver: str = "pressure-based"
+
+
+def _shared_key(kw: dict) -> str:
+    """Case settings other than the boundary conditions: jobs that agree on them can share one
+    solver (one viscosity, one set of relaxation factors and tolerances)."""
+    rest = {k: v for k, v in kw.items() if k not in ("boundaries", "backend")}
+    return json.dumps(rest, sort_keys=True, default=str)
+
+
+def plan_batch(jobs: Sequence[BatchJob], max_cells: int = 50_000_000) -> dict:
+    """Groups of jobs solved together (same shared settings, at most ``max_cells`` cells per
+    launch, largest first) and each job's cell count; no solving (``batch.py:116-141``)."""
+    from .fluent import read_any_mesh
+
+    names = [j.name for j in jobs]
+    if len(set(names)) != len(names):
+        raise ValueError("job names must be unique")
+    rows, meshes, errors = [], {}, {}
+    for j in jobs:
+        try:
+            if j.solver != "pressure-based":
+                raise ValueError(f"solver {j.solver!r} is not available (pressure-based only)")
+            meshes[j.name] = read_any_mesh(j.mesh)[0]
+            rows.append({"name": j.name, "cells": meshes[j.name].n_cells, "key": _shared_key(j.case)})
+        except Exception as exc:  # reported per job by run_batch
+            errors[j.name] = f"{type(exc).__name__}: {exc}"
+    groups: List[List[str]] = []
+    by_key: Dict[str, List[dict]] = {}
+    for r in rows:
+        by_key.setdefault(r["key"], []).append(r)
+    for key, rs in by_key.items():
+        cur, size = [], 0
+        for r in sorted(rs, key=lambda r: -r["cells"]):
+            if cur and size + r["cells"] > max_cells:
+                groups.append(cur)
+                cur, size = [], 0
+            cur.append(r["name"])
+            size += r["cells"]
+        if cur:
+            groups.append(cur)
+    return {"groups": groups, "jobs": [{k: v for k, v in r.items() if k != "key"} for r in rows],
+            "errors": errors, "_meshes": meshes}
+
+
+def run_batch(jobs: Sequence[BatchJob], precision: str = "fp64", device="cuda", use_graph: bool = True,
+              max_cells: int = 50_000_000, log: Callable[[str], None] = print) -> dict:
+    """Solve every job (``runtime/batch.py:144-222``). Instead of one process per case, each group
+    of compatible jobs runs as one solver on the concatenation of its meshes (every kernel one
+    launch over all of them, outer iterations replayed from a HIP graph); closed cases get a
+    pressure reference cell each. Returns ``{"plan", "results" (job order), "wall_seconds"}``;
+    a job that cannot be loaded or solved gets an ``error`` entry and the others go on."""
+    from .case import BoundarySpec, UnstructuredCase, default_boundaries
+
+    t0 = time.perf_counter()
+    plan = plan_batch(jobs, max_cells)
+    meshes = plan.pop("_meshes")
+    by_name = {j.name: j for j in jobs}
+    results: Dict[str, dict] = {n: {"name": n, "mesh": by_name[n].mesh, "error": e}
+                                for n, e in plan["errors"].items()}
+    log(f"Batch: {len(jobs)} cases in {len(plan['groups'])} launch group(s) on {device}")
+    for gi, group in enumerate(plan["groups"]):
+        tg = time.perf_counter()
+        try:
+            kw = {k: v for k, v in by_name[group[0]].case.items() if k not in ("boundaries", "backend")}
+            bnd, closed = {}, []
+            for k, name in enumerate(group):
+                m = meshes[name]
+                b = default_boundaries(m)
+                for z, spec in (by_name[name].case.get("boundaries") or {}).items():
+                    if z not in b:
+                        raise ValueError(f"{name}: zone {z!r} is not a boundary zone of {by_name[name].mesh}")
+                    b[z] = spec if isinstance(spec, BoundarySpec) else BoundarySpec(**spec)
+                bnd.update({f"c{k}/{z}": v for z, v in b.items()})
+                if not any(v.kind in ("outlet", "pressure") for v in b.values()):
+                    closed.append(k)
+            cat = concatenate_meshes([meshes[n] for n in group], [f"c{k}/" for k in range(len(group))])
+            offs = np.concatenate([[0], np.cumsum([meshes[n].n_cells for n in group])])
+            kw.setdefault("precision", precision)
+            solver = SimpleSolver(cat, UnstructuredCase(boundaries=bnd, **kw), device)
+            solver.use_graph = use_graph
+            if closed:
+                solver.set_reference_cells(offs[closed])
+            t1 = time.perf_counter()
+            while not solver.finished:
+                solver.step()
+            solve_s = time.perf_counter() - t1
+            forces = solver.wall_forces()
+            mb = solver.mass_balance()
+            heat = solver.heat_flow() if solver.energy else {}
+            fields = solver.fields()
+            centres = solver.cell_centres()
+            for k, name in enumerate(group):
+                pre = f"c{k}/"
+                strip = lambda d: {z[len(pre):]: v for z, v in d.items() if z.startswith(pre)}  # noqa: E731
+                a, b = int(offs[k]), int(offs[k + 1])
+                res = {"name": name, "mesh": by_name[name].mesh, "error": None, "cells": b - a,
+                       "device": f"{solver.device} (batched launch {gi}, {len(group)} cases)",
+                       "iterations": solver.iteration, "converged": bool(solver.converged),
+                       "solve_seconds": solve_s, "residuals": solver.residuals()["scaled"],
+                       "mass_balance": strip(mb), "wall_forces": strip(forces), "heat_flow": strip(heat),
+                       "stability_rollbacks": len(solver.guard_events)}
+                job = by_name[name]
+                if job.out_dir:
+                    os.makedirs(job.out_dir, exist_ok=True)
+                    out = {f: v[a:b] for f, v in fields.items() if v.shape[0] == solver.nc}
+                    path = os.path.join(job.out_dir, name + ".npz")
+                    np.savez_compressed(path, centres=centres[a:b], **out)
+                    res["npz"] = path
+                res["seconds"] = time.perf_counter() - tg
+                results[name] = res
+            log(f"  group {gi}: {len(group)} case(s), {solver.nc} cells, {solver.iteration} iterations, "
+                f"converged={solver.converged}, {solve_s:.2f} s")
+        except Exception as exc:  # one failing group must not stop the batch
+            for name in group:
+                results[name] = {"name": name, "mesh": by_name[name].mesh, "error": f"{type(exc).__name__}: {exc}",
+                                 "seconds": time.perf_counter() - tg}
+    return {"plan": plan, "results": [results[j.name] for j in jobs], "wall_seconds": time.perf_counter() - t0}
+
+
 _TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "autotune.json")
 
 
@@ -207,4 +337,5 @@ def autotune_grid_cap(solver: SimpleSolver, candidates=(1024, 2048, 4096, 8192),
     return best
 
 
-__all__ = ["concatenate_meshes", "CavitySpec", "sample_cavities", "BatchSolver", "autotune_grid_cap"]
+__all__ = ["concatenate_meshes", "CavitySpec", "sample_cavities", "BatchSolver", "autotune_grid_cap",
+           "BatchJob", "plan_batch", "run_batch"]

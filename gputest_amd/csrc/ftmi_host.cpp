@@ -1,5 +1,6 @@
 // Host-side set-up helpers of libftmi (plain C++; called once per mesh or per AMG rebuild).
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -124,6 +125,63 @@ int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_
         ++na;
     }
     return na;
+}
+
+
+// ---------------------------------------------------------------- Fluent mesh text sections
+// Bulk number parsing for the ASCII sections of a Fluent mesh file (the section structure is
+// scanned in Python). Tokens are separated by blanks, tabs and line breaks.
+
+static inline bool is_blank(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; }
+
+// Hexadecimal integers (face and cell records); returns the count, or -1 - k when token k is
+// not a hexadecimal number, or -2 - cap when there are more than `cap` tokens.
+int64_t ftmi_parse_hex(const char* text, int64_t len, int64_t* out, int64_t cap) {
+    int64_t k = 0, p = 0;
+    while (true) {
+        while (p < len && is_blank(text[p])) ++p;
+        if (p >= len) break;
+        if (k >= cap) return -2 - cap;
+        int64_t v = 0;
+        const int64_t start = p;
+        while (p < len && !is_blank(text[p])) {
+            const char c = text[p];
+            int d;
+            if (c >= '0' && c <= '9')
+                d = c - '0';
+            else if (c >= 'a' && c <= 'f')
+                d = c - 'a' + 10;
+            else if (c >= 'A' && c <= 'F')
+                d = c - 'A' + 10;
+            else
+                return -1 - k;
+            v = v * 16 + d;
+            ++p;
+        }
+        if (p - start > 15) return -1 - k;  // beyond 60 bits: not a mesh index
+        out[k++] = v;
+    }
+    return k;
+}
+
+// Decimal floating-point numbers (node coordinates); same return convention.
+int64_t ftmi_parse_f64(const char* text, int64_t len, double* out, int64_t cap) {
+    int64_t k = 0, p = 0;
+    std::vector<char> tok;
+    while (true) {
+        while (p < len && is_blank(text[p])) ++p;
+        if (p >= len) break;
+        if (k >= cap) return -2 - cap;
+        const int64_t start = p;
+        while (p < len && !is_blank(text[p])) ++p;
+        tok.assign(text + start, text + p);
+        tok.push_back('\0');
+        char* end = nullptr;
+        const double v = std::strtod(tok.data(), &end);
+        if (end != tok.data() + (p - start)) return -1 - k;
+        out[k++] = v;
+    }
+    return k;
 }
 
 }  // extern "C"

@@ -176,6 +176,12 @@ int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* id
 int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
                                      int32_t* agg, int32_t* size);
 
+/* new: bulk number parsing for the ASCII sections of a Fluent mesh file (host only; the
+ * reference parses them in Python, flowtrace/mesh/fluent_msh.py:151-156 and 260). Blank-separated
+ * tokens; returns the count, -1 - k if token k is malformed, -2 - cap if there are more than cap. */
+int64_t ftmi_parse_hex(const char* text, int64_t len, int64_t* out, int64_t cap);
+int64_t ftmi_parse_f64(const char* text, int64_t len, double* out, int64_t cap);
+
 /* new: SA-AMG set-up on the device (amg.py:60-117, everything but the aggregation) */
 /* CSR data of the pressure matrix from its face form (he_slot: CSR slot of each half-edge) */
 int ftmi_amgs_ldu_to_csr(int64_t n, const int32_t* cptr, const int32_t* he_slot, const int32_t* slot_diag,

@@ -1,0 +1,104 @@
+"""Checkpoints taken from a running GPU solve (asynchronous pinned snapshots) and the batch
+job-list driver (``flowtrace/runtime/checkpoint.py``, ``flowtrace/runtime/batch.py``) on the GPU:
+the snapshot equals the synchronous state at the moment it was taken although the solve ran
+on, a restart from the file matches the oracle restarted from the same file, and every case of
+a mixed batch (closed cavities and the square cylinder in one launch) reaches its own steady
+solution and the reference's drag pin."""
+import numpy as np
+import pytest
+import torch
+
+import oracle.simplec_oracle as O
+from gputest_amd import case as C
+from gputest_amd.batch import BatchJob, run_batch
+from gputest_amd.checkpoint import CheckpointWriter, load_checkpoint
+from gputest_amd.fluent import read_any_mesh, write_fluent_mesh
+from gputest_amd.mesh import cartesian_mesh_2d
+from gputest_amd.simple import SimpleSolver
+
+from fluent_cases import MESH, PINS, drag_lift
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def _cavity(mod, tol=0.0, max_iter=10 ** 6):
+    B = mod.Boundary
+    return mod.Case(boundaries={"top": B("wall", velocity=[1.0, 0.0])}, viscosity=0.01, tol=tol, max_iter=max_iter)
+
+
+def test_snapshot_checkpoint_and_restart(tmp_path):
+    mesh = cartesian_mesh_2d(40, 40, triangulate=True, warp=0.05)
+    s = SimpleSolver(mesh, _cavity(C), "cuda")
+    for _ in range(10):
+        s.step()
+    w = CheckpointWriter(str(tmp_path), keep=2)
+    w.submit("it10", s, {"iteration": s.iteration})  # asynchronous copy into pinned memory
+    for _ in range(5):  # the solve runs on while the file is written
+        s.step()
+    w.submit("it15", s, {"iteration": s.iteration})
+    ref15 = s.state_dict()
+    for _ in range(3):
+        s.step()
+    w.submit("it18", s, {"iteration": s.iteration})  # reuses the it10 buffers
+    w.close()
+    assert w.last_error is None
+    st15, meta = load_checkpoint(str(tmp_path / "it15.npz"))
+    assert meta == {"iteration": 15} and st15["iteration"] == 15
+    for k in ("U", "p", "F", "Fb"):
+        np.testing.assert_array_equal(st15[k], ref15[k])
+    assert not (tmp_path / "it10.npz").exists()  # pruned to the newest two
+    # restart from the file: the HIP solver and the oracle continue alike
+    g = SimpleSolver(mesh, _cavity(C), "cuda")
+    g.load_state(st15)
+    o = O.Simplec(mesh, _cavity(O))
+    o.load_state(st15)
+    for _ in range(20):
+        g.step()
+        o.step()
+    f = g.fields()
+    for k, ref in (("x-velocity", o.U[:, 0]), ("y-velocity", o.U[:, 1])):
+        assert rel_l2(f[k], ref) < 1e-8, k
+
+
+def test_run_batch_solves_each_case(tmp_path):
+    # two closed cavities (a lid-speed override each) and the open square-cylinder channel,
+    # all at nu = 0.1: one launch group
+    paths = []
+    for n, tri in ((24, False), (20, True)):
+        p = str(tmp_path / f"cav{n}.msh")
+        write_fluent_mesh(cartesian_mesh_2d(n, n, triangulate=tri), p)
+        paths.append(p)
+    shared = {"viscosity": 0.1, "tol": 1e-10, "max_iter": 3000}  # converged well past the batch's own stopping noise
+    lid = lambda u: {"boundaries": {"top": {"kind": "wall", "velocity": [u, 0.0]}}, **shared}  # noqa: E731
+    jobs = [BatchJob("cav24", paths[0], lid(1.0), out_dir=str(tmp_path / "out")),
+            BatchJob("cav20", paths[1], lid(2.0), out_dir=str(tmp_path / "out")), BatchJob("sqcyl", MESH, dict(shared)),
+            BatchJob("missing", str(tmp_path / "none.msh"), dict(shared))]
+    out = run_batch(jobs, log=lambda m: print(m, flush=True))
+    assert out["plan"]["groups"] == [["sqcyl", "cav20", "cav24"]]  # largest first (800 triangles, 576 quads)
+    res = {r["name"]: r for r in out["results"]}
+    assert res["missing"]["error"] and "FileNotFoundError" in res["missing"]["error"]
+    for name in ("cav24", "cav20", "sqcyl"):
+        assert res[name]["error"] is None and res[name]["converged"], res[name]
+    cd, cl = drag_lift(res["sqcyl"]["wall_forces"])
+    assert abs(cd - PINS[0][1]) < 2e-4 and abs(cl) < 1e-6
+    mb = res["sqcyl"]["mass_balance"]
+    assert {"inlet", "outlet"} <= set(mb) and abs(sum(mb.values())) < 1e-8 * abs(mb["inlet"])
+    # each cavity against its own oracle solve
+    for name, path, u in (("cav24", paths[0], 1.0), ("cav20", paths[1], 2.0)):
+        z = np.load(res[name]["npz"])
+        mesh = read_any_mesh(path)[0]
+        B = O.Boundary
+        o = O.Simplec(mesh, O.Case(boundaries={"top": B("wall", velocity=[u, 0.0]), "left": B("wall"),
+                                               "right": B("wall"), "bottom": B("wall")}, **shared))
+        while not o.finished:
+            o.step()
+        assert z["centres"].shape == (mesh.n_cells, 2)
+        for k, ref in (("x-velocity", o.U[:, 0]), ("y-velocity", o.U[:, 1])):
+            e = rel_l2(z[k], ref)
+            print(name, k, e)
+            assert e < 1e-7, (name, k, e)
+    print({n: (r.get("iterations"), r.get("seconds")) for n, r in res.items()}, out["wall_seconds"])
