@@ -79,6 +79,11 @@ _SIGS = {
     "ftmi_cast_f64_f32": ([I64, P, P, P], I32),
     "ftmi_amg_aggregate_host": ([I64, P, P, P], I64),
     "ftmi_amg_aggregate_mask_host": ([I64, P, P, P, P, P], I64),
+    "ftmi_diag_rows": ([I64, P, P, P, P, P, P, F64, P, P, P, P, P], I32),
+    "ftmi_diag_cell_flux": ([I64, P, P, P, P, P, P, P, P, P], I32),
+    "ftmi_diag_work_doubles": ([], I64),
+    "ftmi_diag_rowset": ([I64, I32, P, P, F64, P, P, P], I32),
+    "ftmi_diag_vstats": ([I64, I32, I64, P, P, P, P, P], I32),
     "ftmi_parse_hex": ([P, I64, P, I64], I64),
     "ftmi_parse_f64": ([P, I64, P, I64], I64),
     "ftmi_amgs_ldu_to_csr": ([I64] + [P] * 8, I32),

@@ -185,9 +185,11 @@ class SimpleSolver:
         self.Fst, self.cf, self.Fbst, self.cb = z(nf), z(nf), z(nb), z(nb)
         self.diag_p, self.offp, self.mass, self.rhs_p, self.pp = z(nc), z(2 * nf), z(nc), z(nc), z(nc)
         self.gpp = z(nc, dim)
+        # the iteration-start fields, kept for the diagnostics of the latest iteration
+        self.U_old, self.p_old = z(dim, nc), z(nc)
         if self.energy:
             self.Tb, self.gT, self.Tnew, self.bT = z(nb), z(nc, dim), z(nc), z(nc)
-            self.diagT, self.diagT_r = z(nc), z(nc)
+            self.diagT, self.diagT_r, self.offT, self.T_old = z(nc), z(nc), z(2 * nf), z(nc)
         if self.mixed:
             f = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
             self.diag32, self.diag_r32, self.inv_r32, self.off32 = f(nc), f(nc), f(nc), f(2 * nf)
@@ -265,6 +267,8 @@ class SimpleSolver:
         L, s = nat.lib(), nat.stream_ptr()
         M, B, R = ctypes.byref(fv.c_mesh), ctypes.byref(self.c_bc), ctypes.byref(self.c_red)
         rho, mu = c.density, c.viscosity
+        self.U_old.copy_(self.U)
+        self.p_old.copy_(self.p)
         # 1-2. boundary values with the lagged gradients, then the new gradients
         lag = self._has_lag
         nat.check(L.ftmi_simplec_bvals(M, B, nat.ptr(self.U), nat.ptr(self.p), nat.ptr(self.gU) if lag else None,
@@ -377,7 +381,9 @@ class SimpleSolver:
             self.last_linear["pressure_rounds"] = pinfo.rounds
         if tinfo is not None:
             self.last_linear["energy_iters"] = tinfo.iterations
-        self._capture = {"mom_info": infos, "p_info": pinfo, "energy_info": tinfo}
+        # what the diagnostics of this iteration need besides the device systems it left behind
+        self._capture = {"mom_info": infos, "p_info": pinfo, "energy_info": tinfo, "iteration": self.iteration,
+                         "alpha_u": float(self.relax_u), "alpha_T": float(c.relax_T), "rolled_back": False}
         prev = getattr(self, "_last_res", {}), getattr(self, "_last_raw", {})
         self._last_raw = dict(res)
         if self.iteration <= 5:
@@ -387,6 +393,7 @@ class SimpleSolver:
         self._last_res = res
         rolled_back = self._stability_guard(self._last_raw["continuity"], flux_scale)
         if rolled_back:
+            self._capture["rolled_back"] = True
             self._graph = None  # relaxation and fields changed under the recorded graph
             self._last_res, self._last_raw = prev
             return 1.0
@@ -396,6 +403,7 @@ class SimpleSolver:
 
     def _energy_step(self, L, s, M, B, R):
         c = self.case
+        self.T_old.copy_(self.T)
         nat.check(L.ftmi_energy_bvals(M, B, nat.ptr(self.T), nat.ptr(self.gT) if self._has_lagT else None,
                                       nat.ptr(self.Fb), nat.ptr(self.Tb), s), "energy bvals")
         arr = ctypes.c_void_p * 1
@@ -407,7 +415,7 @@ class SimpleSolver:
         a.phi[0], a.grad[0], a.phib[0] = nat.ptr(self.T), nat.ptr(self.gT), nat.ptr(self.Tb)
         a.b[0], a.x0[0] = nat.ptr(self.bT), nat.ptr(self.Tnew)
         a.relax, a.second_order, a.simplec, a.cp = float(c.relax_T), int(c.scheme == "second-order"), -1, float(c.cp)
-        a.diag, a.diag_r, a.off = nat.ptr(self.diagT), nat.ptr(self.diagT_r), nat.ptr(self.off)
+        a.diag, a.diag_r, a.off = nat.ptr(self.diagT), nat.ptr(self.diagT_r), nat.ptr(self.offT)
         a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(self.inv_r)
         if self.mixed:
             self._fp32_outputs(a, 1)
@@ -416,7 +424,7 @@ class SimpleSolver:
             info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
                                       True, ws=self.ws)
         else:
-            info = bicgstab_ldu(self.fv, self.diagT_r, self.off, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
+            info = bicgstab_ldu(self.fv, self.diagT_r, self.offT, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
                                 ws=self.ws)
         self.T.copy_(self.Tnew)  # a copy, not a swap: recorded graphs keep their pointers
         return info
@@ -523,6 +531,13 @@ class SimpleSolver:
         return False
 
     # ------------------------------------------------------------------ state, fields, reports
+    def diagnostics(self) -> dict:
+        """Algebraic, conservation and health diagnostics of the latest outer iteration, from the
+        systems it left on the device (``simple.py:diagnostics``, ``diagnostics.py:214-360``)."""
+        from .diagnostics import evaluate
+
+        return evaluate(self)
+
     def residuals(self) -> dict:
         r = dict(getattr(self, "_last_res", {}))
         raw = dict(getattr(self, "_last_raw", r))

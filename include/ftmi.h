@@ -176,6 +176,26 @@ int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* id
 int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
                                      int32_t* agg, int32_t* size);
 
+/* new: device reductions of the outer-iteration diagnostics (flowtrace/unstructured/diagnostics.py:
+ * 103-181 _rows_of_system / _row_set, 214-290 evaluate); results land in device doubles.
+ * rows: r = b' - A x, s = |b'| + |A||x| of an LDU system, b' = b - c dgb xb when dgb != NULL */
+int ftmi_diag_rows(int64_t n, const double* dg, const int32_t* ptr, const int32_t* nbr, const double* off,
+                   const double* b, const double* dgb, double c, const double* xb, const double* x, double* r,
+                   double* s, void* stream);
+/* per cell: m = net outflow of (F, Fb), t = sum |flux| (cface = f << 1 | is-neighbour) */
+int ftmi_diag_cell_flux(int64_t n, const int32_t* cptr, const int32_t* cface, const int32_t* bptr,
+                        const int32_t* bface, const double* F, const double* Fb, double* m, double* t, void* stream);
+/* doubles of work for the two reductions below */
+int64_t ftmi_diag_work_doubles(void);
+/* a row set of ncomp (<= 4) (r, s) vectors: out[0] = s_ref, out[1 + 9 c + j] = row-scaled max,
+ * its row (-1 if none), active, excluded, excluded max |r|, non-finite, l1, max, l2 */
+int ftmi_diag_rowset(int64_t n, int ncomp, const double* const* r, const double* const* s, double gamma_k,
+                     double* out, double* work, void* stream);
+/* a vector field (ncomp components stride apart): out[0..7] = max row norm of x - y (y may be
+ * NULL), max row norm of x, min, max, non-finite entries, sum, sum |x|, sum x^2 */
+int ftmi_diag_vstats(int64_t n, int ncomp, int64_t stride, const double* x, const double* y, double* out,
+                     double* work, void* stream);
+
 /* new: bulk number parsing for the ASCII sections of a Fluent mesh file (host only; the
  * reference parses them in Python, flowtrace/mesh/fluent_msh.py:151-156 and 260). Blank-separated
  * tokens; returns the count, -1 - k if token k is malformed, -2 - cap if there are more than cap. */

@@ -610,12 +610,13 @@ class Simplec:
         diag_r = diag / au
         Ar = fv.matrix(diag_r, aPN, aNP)
         Ustar = np.empty_like(U)
-        its, brs = [], []
+        its, brs, infos = [], [], []
         for k in range(dim):
             b = rhs[k] + ((1.0 - au) / au) * diag * U[:, k]
             x, info = self._solve(Ar, b, U[:, k], diag_r)
             Ustar[:, k] = x
             its.append(info.iterations)
+            infos.append(info)
             brs.append(b)
         if self.algo == "SIMPLEC":
             denom = diag_r - fv.scatter(-aPN, -aNP)
@@ -652,7 +653,8 @@ class Simplec:
         self.U = Ustar - dP[:, None] * gpp
         self._capture = {"A": A, "b": rhs, "Ar": Ar, "br": brs, "U_star": Ustar, "mom_iters": its, "Fst": Fst,
                          "Fbst": Fbst, "mass": mass, "Ap": Ap, "pp": pp, "p_iters": pinfo.iterations, "dP": dP,
-                         "gp": gp, "gU": gU}
+                         "gp": gp, "gU": gU, "U_old": U, "p_old": p, "F": self.F, "Fb": self.Fb, "U_new": self.U,
+                         "p_new": self.p, "alpha_u": au, "mom_info": infos, "p_info": pinfo}
         if self.energy:
             res["energy"] = self._energy_step()
         self.iteration += 1
@@ -665,7 +667,10 @@ class Simplec:
         flux_scale = float(np.sum(np.abs(self.F))) + float(np.sum(np.abs(self.Fb)))
         res["continuity"] /= max(self._res_max5["continuity"], 1e-3 * flux_scale)
         self._last_res = res
-        if self._guard(self._last_raw["continuity"], flux_scale):
+        self._capture["iteration"] = self.iteration
+        rolled_back = self._guard(self._last_raw["continuity"], flux_scale)
+        self._capture["rolled_back"] = rolled_back
+        if rolled_back:
             self._last_res, self._last_raw = prev
             return
         if self.iteration > 5 and max(res.values()) < c.tol:
@@ -695,7 +700,9 @@ class Simplec:
         dr = diag / aT
         Ar = fv.matrix(dr, aPN, aNP)
         self.T, info = self._solve(Ar, rhs + ((1.0 - aT) / aT) * diag * T, T, dr, tol=1e-3, maxiter=60)
-        self._capture.update({"T_A": A, "T_b": rhs, "T_iters": info.iterations})
+        self._capture.update({"T_A": A, "T_b": rhs, "T_iters": info.iterations,
+                              "energy": {"A": A, "b": rhs, "T_old": T, "Ar": Ar, "br": rhs + ((1.0 - aT) / aT) * diag * T,
+                                         "T_new": self.T, "alpha": aT, "info": info}})
         return r
 
     # --- stability guard (simple.py:705-776) ---
