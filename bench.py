@@ -210,6 +210,7 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
         s.timers[k] = 0.0
     s.amg_setup_seconds = 0.0
     s.amg_rebuilds = 0
+    s.amg_refreshes = 0
     s.amg_timing = {}
     guard0 = len(s.guard_events)
     pits = []
@@ -222,10 +223,11 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
     dt = time.perf_counter() - t0
     guards = len(s.guard_events) - guard0
     print(f"[bench:{tag}] timed {steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
-          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds; guard events {guards}; "
+          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds + {s.amg_refreshes} refreshes; guard events {guards}; "
           f"pressure iterations {pits}; last {s.last_linear}", file=sys.stderr, flush=True)
     info = {"stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
             "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
+            "amg_refreshes": s.amg_refreshes,
             "amg_setup_host_seconds": {k: round(v, 4) for k, v in s.amg_timing.items()},
             "guard_events": guards, "pressure_iters_total": int(sum(p or 0 for p in pits))}
     return s, dt, info

@@ -77,6 +77,7 @@ _SIGS = {
     "ftmi_bicgstab_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
     "ftmi_pcg_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, P, I32, I32, P, P, P, P, P], I32),
     "ftmi_cast_f64_f32": ([I64, P, P, P], I32),
+    "ftmi_gather_f64": ([I64, P, P, P, P], I32),
     "ftmi_amg_aggregate_host": ([I64, P, P, P], I64),
     "ftmi_amg_aggregate_mask_host": ([I64, P, P, P, P, P], I64),
     "ftmi_diag_rows": ([I64, P, P, P, P, P, P, F64, P, P, P, P, P], I32),

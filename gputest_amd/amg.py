@@ -126,12 +126,17 @@ class SmoothedAggregationAMG:
         return cls(A0, fv.device, singular=singular, host_pattern=(fv.indptr_h, fv.indices_h), **kw)
 
     # ------------------------------------------------------------------ set-up
-    def _spgemm(self, X: DevCSR, Y: DevCSR, xscale=None, agg=None, tval=None, omega_num=None, omega_scale=0.0):
+    def _spgemm(self, X: DevCSR, Y: DevCSR, xscale=None, agg=None, tval=None, omega_num=None, omega_scale=0.0,
+                keep=None):
+        """C = X Y (count pass, scan, fill pass). With ``keep`` (a dict) the count-pass work is kept
+        there, so ``_spgemm_fill`` can recompute the values of the same pattern later."""
         L, st, dev = nat.lib(), nat.stream_ptr(), self.device
         n = X.n
         cnt = torch.zeros(n, dtype=torch.int32, device=dev)  # rows over the hash capacity stay 0
         long_rows = torch.empty(n, dtype=torch.int32, device=dev)
         n_long = torch.zeros(1, dtype=torch.int32, device=dev)
+        if keep is not None:
+            keep.update(cnt=cnt, long_rows=long_rows, n_long=n_long)
         args = (n, Y.ncols, nat.ptr(X.ptr), nat.ptr(X.col), nat.ptr(X.val), nat.ptr(Y.ptr), nat.ptr(Y.col),
                 nat.ptr(Y.val),
                 nat.ptr(xscale), nat.ptr(agg), nat.ptr(tval), nat.ptr(omega_num), float(omega_scale))
@@ -147,6 +152,15 @@ class SmoothedAggregationAMG:
             o = self._ovf.cpu().numpy()
             print(f"[amg] spgemm rows {n} nnz {nnz} overflow {o.tolist()}", flush=True)
         return DevCSR(n, Y.ncols, ptr, col, val)
+
+    def _spgemm_fill(self, X: DevCSR, Y: DevCSR, C: DevCSR, keep: dict, xscale=None, agg=None, tval=None,
+                     omega_num=None, omega_scale=0.0):
+        """Values of C = X Y into C's existing pattern (the fill pass alone; no host read)."""
+        args = (X.n, Y.ncols, nat.ptr(X.ptr), nat.ptr(X.col), nat.ptr(X.val), nat.ptr(Y.ptr), nat.ptr(Y.col),
+                nat.ptr(Y.val), nat.ptr(xscale), nat.ptr(agg), nat.ptr(tval), nat.ptr(omega_num), float(omega_scale))
+        nat.check(nat.lib().ftmi_amgs_spgemm(1, *args, nat.ptr(keep["cnt"]), nat.ptr(C.ptr), nat.ptr(C.col),
+                                             nat.ptr(C.val), nat.ptr(self._ovf), nat.ptr(keep["long_rows"]),
+                                             nat.ptr(keep["n_long"]), nat.stream_ptr()), "spgemm fill")
 
     def _scan(self, cnt):
         n = cnt.numel()
@@ -179,7 +193,8 @@ class SmoothedAggregationAMG:
         self._ovf = torch.zeros(4, dtype=torch.int32, device=dev)
         self._bsum = torch.empty(2048, dtype=torch.int64, device=dev)
         partials = f64(2048 * 8)
-        self.levels, self.aggregates, self.rhos = [], [], []
+        self.levels, self.aggregates, self.rhos, self._plans = [], [], [], []
+        self._partials = partials
         pattern = host_pattern
         while A.n > self.max_coarse and len(self.levels) < self.max_levels - 1:
             n = A.n
@@ -226,9 +241,15 @@ class SmoothedAggregationAMG:
             nat.check(L.ftmi_amgs_tentative(n, nat.ptr(agg), nat.ptr(size), nat.ptr(t_ptr), nat.ptr(t_val), st),
                       "tentative")
             T = DevCSR(n, na, t_ptr, agg, t_val)
-            P = self._spgemm(A, T, xscale=dinv, agg=agg, tval=t_val, omega_num=rho, omega_scale=4.0 / 3.0)
+            plan = {"T": T, "diag": diag, "y": y, "rho": rho, "kP": {}, "kAP": {}, "kRAP": {}}
+            P = self._spgemm(A, T, xscale=dinv, agg=agg, tval=t_val, omega_num=rho, omega_scale=4.0 / 3.0,
+                             keep=plan["kP"])
             R = self._transpose(P)
-            Ac = self._spgemm(R, self._spgemm(A, P))
+            plan["perm"] = self._transpose_perm(P, R)
+            AP = self._spgemm(A, P, keep=plan["kAP"])
+            Ac = self._spgemm(R, AP, keep=plan["kRAP"])
+            plan["AP"] = AP
+            self._plans.append(plan)
             self.levels.append((A, P, R, dinv))
             self.aggregates.append(agg_h.astype(np.int64))  # a copy: the pinned buffer is recycled
             self.rhos.append(rho)
@@ -238,17 +259,74 @@ class SmoothedAggregationAMG:
             raise RuntimeError(f"AMG set-up: {ovf[0]} sparse-product rows exceeded the LDS hash capacity "
                                f"(max distinct {ovf[1]}, max candidates {ovf[2]}, e.g. row {ovf[3]})")
         self.coarse = A
-        Ad = A.to_scipy().toarray()
-        if self.singular:
-            Ad = Ad + 1e-10 * np.abs(np.diag(Ad)).mean() * np.eye(Ad.shape[0])
-        try:
-            cinv = np.linalg.inv(Ad)
-        except np.linalg.LinAlgError:
-            cinv = np.linalg.pinv(Ad)
-        self.coarse_n = Ad.shape[0]
+        cinv = self._coarse_inverse()
+        self.coarse_n = cinv.shape[0]
         self.coarse_inv_h = cinv
         self._upload(cinv)
         self.timing["total"] = clock() - t_start
+
+    def _coarse_inverse(self) -> np.ndarray:
+        Ad = self.coarse.to_scipy().toarray()
+        if self.singular:
+            Ad = Ad + 1e-10 * np.abs(np.diag(Ad)).mean() * np.eye(Ad.shape[0])
+        try:
+            return np.linalg.inv(Ad)
+        except np.linalg.LinAlgError:
+            return np.linalg.pinv(Ad)
+
+    def _transpose_perm(self, P: DevCSR, R: DevCSR) -> torch.Tensor:
+        """perm with R.val[j] = P.val[perm[j]]: the transpose applied to the entry numbers."""
+        L, dev = nat.lib(), self.device
+        idx = torch.arange(P.nnz, dtype=torch.float64, device=dev)
+        fill = torch.zeros(P.ncols, dtype=torch.int32, device=dev)
+        rcol = torch.empty(P.nnz, dtype=torch.int32, device=dev)
+        rnum = torch.empty(P.nnz, dtype=torch.float64, device=dev)
+        nat.check(L.ftmi_amgs_transpose_fill(P.n, P.ncols, nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(idx),
+                                             nat.ptr(R.ptr), nat.ptr(fill), nat.ptr(rcol), nat.ptr(rnum),
+                                             nat.stream_ptr()), "transpose perm")
+        return rnum.to(torch.int64)
+
+    # ------------------------------------------------------------------ numeric refresh
+    def refresh(self):
+        """Recompute every value of the hierarchy for new level-0 values (already in
+        ``levels[0][0].val``), keeping the aggregates and all sparsity patterns: D^-1, rho, the
+        smoothed prolongator, R, A P and R (A P) by the fill passes alone, and the coarse inverse.
+        No aggregation and no host read except the small coarse matrix. The result is a valid
+        smoothed-aggregation preconditioner of the new matrix built on the old aggregates (not
+        the reference's hierarchy, which re-aggregates)."""
+        L, st = nat.lib(), nat.stream_ptr()
+        for (A, P, R, dinv), plan in zip(self.levels, self._plans):
+            n = A.n
+            nat.check(L.ftmi_amgs_diag(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(plan["diag"]),
+                                       nat.ptr(dinv), st), "diag")
+            x = start_vector_device(n, self.device)
+            nat.check(L.ftmi_amgs_power(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv),
+                                        nat.ptr(x), nat.ptr(plan["y"]), 15, nat.ptr(self._partials),
+                                        nat.ptr(plan["rho"]), st), "power")
+            T = plan["T"]
+            self._spgemm_fill(A, T, P, plan["kP"], xscale=dinv, agg=T.col, tval=T.val, omega_num=plan["rho"],
+                              omega_scale=4.0 / 3.0)
+            nat.check(L.ftmi_gather_f64(R.nnz, nat.ptr(plan["perm"]), nat.ptr(P.val), nat.ptr(R.val), st), "gather")
+            self._spgemm_fill(A, P, plan["AP"], plan["kAP"])
+            nxt = self._plans_next(A)
+            self._spgemm_fill(R, plan["AP"], nxt, plan["kRAP"])
+        cinv = self._coarse_inverse()
+        self.coarse_inv_h = cinv
+        self._cinv.copy_(torch.as_tensor(np.ascontiguousarray(cinv), device=self.device))
+
+    def _plans_next(self, A):
+        for k, lv in enumerate(self.levels):
+            if lv[0] is A:
+                return self.levels[k + 1][0] if k + 1 < len(self.levels) else self.coarse
+        raise KeyError("level not found")
+
+    def refresh_ldu(self, fv, diag, off):
+        """``refresh`` for a new pressure matrix given in the face (LDU) form of ``fv``."""
+        A0 = self.levels[0][0] if self.levels else self.coarse
+        nat.check(nat.lib().ftmi_amgs_ldu_to_csr(fv.nc, nat.ptr(fv.cptr), nat.ptr(fv.he_slot), nat.ptr(fv.slot_diag),
+                                                 nat.ptr(fv.indptr), nat.ptr(diag), nat.ptr(off), nat.ptr(A0.val),
+                                                 nat.stream_ptr()), "ldu_to_csr")
+        self.refresh()
 
     def _upload(self, cinv):
         dev, L = self.device, nat.lib()
@@ -311,9 +389,10 @@ class AmgFP32:
         L, dev = nat.lib(), src.device
         self.src, self.device = src, dev
         self.handle = L.ftmi_amg_create_f32(float(src.omega), int(src.sweeps))
-        self._keep = []
+        self._keep, self._vals = [], []
         for A, P, R, dinv in src.levels:
             av, pv, rv, dv = to_f32(A.val), to_f32(P.val), to_f32(R.val), to_f32(dinv)
+            self._vals.append((av, pv, rv, dv))
             v = [torch.zeros(A.n, dtype=torch.float32, device=dev) for _ in range(4)]
             self._keep += [av, pv, rv, dv] + v
             nat.check(L.ftmi_amg_add_level_f32(self.handle, A.n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(av),
@@ -326,6 +405,17 @@ class AmgFP32:
         self._cx = torch.zeros(nc, dtype=torch.float32, device=dev)
         nat.check(L.ftmi_amg_set_coarse_f32(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb),
                                             nat.ptr(self._cx)), "amg_set_coarse_f32")
+
+    def refresh(self):
+        """Round the (refreshed) FP64 values of the source into this twin's FP32 buffers."""
+        from .krylov import to_f32
+
+        for (A, P, R, dinv), (av, pv, rv, dv) in zip(self.src.levels, self._vals):
+            to_f32(A.val, av)
+            to_f32(P.val, pv)
+            to_f32(R.val, rv)
+            to_f32(dinv, dv)
+        to_f32(self.src._cinv, self._cinv)
 
     def __call__(self, b: torch.Tensor) -> torch.Tensor:
         b = b.contiguous()

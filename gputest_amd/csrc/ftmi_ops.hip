@@ -47,6 +47,14 @@ __global__ void __launch_bounds__(kBlock) k_cast_f32(int64_t n, const double* __
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) y[i] = (float)x[i];
 }
 
+// y[j] = x[idx[j]]
+__global__ void __launch_bounds__(kBlock) k_gather_f64(int64_t n, const int64_t* __restrict__ idx,
+                                                       const double* __restrict__ x, double* __restrict__ y) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t j = lo + threadIdx.x; j < hi; j += kBlock) y[j] = x[idx[j]];
+}
+
 // y[i, :] = sum_j A[i, j] X[j, :]   (k right-hand sides, CSR order as ft_csr_spmm)
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_csr_spmm(int64_t n, const int32_t* __restrict__ ptr,
@@ -211,6 +219,12 @@ int ftmi_csr_spmm(int64_t nrows, int64_t k, const int32_t* indptr, const int32_t
 int ftmi_cast_f64_f32(int64_t n, const double* x, float* y, void* stream) {
     if (n > 0) k_cast_f32<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, x, y);
     FTMI_LAUNCH_CHECK("ftmi_cast_f64_f32");
+    return 0;
+}
+
+int ftmi_gather_f64(int64_t n, const int64_t* idx, const double* x, double* y, void* stream) {
+    if (n > 0) k_gather_f64<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, x, y);
+    FTMI_LAUNCH_CHECK("ftmi_gather_f64");
     return 0;
 }
 

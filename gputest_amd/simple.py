@@ -104,6 +104,10 @@ class SimpleSolver:
         self.amg_rebuild = 10
         self.amg_setup_seconds = 0.0
         self.amg_rebuilds = 0
+        # mixed precision: numeric refreshes of the FP32 preconditioner between full set-ups
+        self.mixed_amg_refreshes = 5
+        self.amg_refreshes = 0
+        self._amg_refreshed = 0
         self.amg_timing: Dict[str, float] = {}
         self.last_linear = {}
         self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
@@ -329,13 +333,23 @@ class SimpleSolver:
         # 6. pressure correction
         if rebuild and (self._amg is None or self._amg_age >= self.amg_rebuild):
             t0 = time.perf_counter()
-            self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
-            self._amg32 = self._amg.to_fp32() if self.mixed else None
+            if self.mixed and self._amg is not None and self._amg_refreshed < self.mixed_amg_refreshes:
+                # mixed precision: the hierarchy is only the FP32 preconditioner, so its values
+                # are refreshed on the kept aggregates and patterns (no aggregation, no host
+                # reads but the coarse matrix); a full set-up every few refreshes
+                self._amg.refresh_ldu(fv, self.diag_p, self.offp)
+                self._amg32.refresh()
+                self._amg_refreshed += 1
+                self.amg_refreshes += 1
+            else:
+                self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
+                self._amg32 = self._amg.to_fp32() if self.mixed else None
+                self._amg_refreshed = 0
+                for k, v in self._amg.timing.items():
+                    self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
+                self.amg_rebuilds += 1
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
-            for k, v in self._amg.timing.items():
-                self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
-            self.amg_rebuilds += 1
         if self.mixed:
             to_f32(self.offp, self.offp32)
             self.pp.zero_()

@@ -145,6 +145,8 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
                        void* stream);
 /* y = (float) x */
 int ftmi_cast_f64_f32(int64_t n, const double* x, float* y, void* stream);
+/* new: y[j] = x[idx[j]] (values of a transposed pattern in the AMG numeric refresh) */
+int ftmi_gather_f64(int64_t n, const int64_t* idx, const double* x, double* y, void* stream);
 
 /* ---------------------------------------------------------------- smoothed-aggregation AMG
  * replaces SmoothedAggregationAMG (flowtrace/linalg/amg.py:80-146). The hierarchy handle

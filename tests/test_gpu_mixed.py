@@ -137,3 +137,51 @@ def test_mixed_converges_to_the_fp64_oracle_solution():
     assert s.converged
     for k, e in errs.items():
         assert e <= 1e-6, (k, e)
+
+
+def _sa_on_aggregates(A, aggregates, singular=False):
+    """Host smoothed-aggregation hierarchy of A on given aggregates (``amg.py:80-117`` with the
+    aggregation step replaced): the reference for a numeric refresh."""
+    levels = []
+    A = sp.csr_matrix(A)
+    for agg in aggregates:
+        n, na = A.shape[0], int(agg.max()) + 1
+        T = sp.csr_matrix((np.ones(n), (np.arange(n), agg)), shape=(n, na))
+        T = T @ sp.diags(1.0 / np.sqrt(np.asarray(T.sum(axis=0)).ravel()))
+        Dinv = 1.0 / A.diagonal()
+        DA = sp.diags(Dinv) @ A
+        P = (T - (4.0 / 3.0 / O.spectral_radius(DA)) * (DA @ T)).tocsr()
+        levels.append((A, P, P.T.tocsr(), Dinv))
+        A = (P.T @ A @ P).tocsr()
+    M = O.SAAMG.__new__(O.SAAMG)
+    M.omega, M.sweeps, M.levels = 0.8, 2, levels
+    Ad = A.toarray()
+    if singular:
+        Ad = Ad + 1e-10 * np.abs(np.diag(Ad)).mean() * np.eye(Ad.shape[0])
+    M.coarse_inv = np.linalg.inv(Ad)
+    return M
+
+
+def test_numeric_refresh_matches_the_hierarchy_on_the_kept_aggregates():
+    """A hierarchy set up on one matrix, refreshed with another (fill passes on the kept
+    patterns, no aggregation), against the host smoothed-aggregation hierarchy of the second
+    matrix on the same aggregates: every level's operator and prolongator, and the V-cycle."""
+    fv = FVMesh(cartesian_mesh_2d(48, 40, triangulate=True, warp=0.05), DEV)
+    A1, d1, off1 = _ldu_system(fv, 7, symmetric=True)
+    A2, d2, off2 = _ldu_system(fv, 8, symmetric=True)
+    amg = SmoothedAggregationAMG.from_ldu(fv, t(d1), t(off1))
+    amg32 = amg.to_fp32()
+    amg.refresh_ldu(fv, t(d2), t(off2))
+    amg32.refresh()
+    ref = _sa_on_aggregates(A2, amg.aggregates)
+    assert len(ref.levels) == len(amg.levels)
+    for (A, Pm, R, dinv), (Ar, Pr, Rr, dr) in zip(amg.levels, ref.levels):
+        for x, y in ((A, Ar), (Pm, Pr), (R, Rr)):
+            diff = abs(x.to_scipy() - y).max()
+            assert diff <= 1e-12 * abs(y).max(), diff
+        np.testing.assert_allclose(h(dinv), dr, rtol=1e-14)
+    bvec = np.random.default_rng(9).standard_normal(fv.nc)
+    z = h(amg(t(bvec)))
+    assert rel_l2(z, ref(bvec)) < 1e-11
+    z32 = h(amg32(to_f32(t(bvec))))
+    assert rel_l2(z32, z) < 1e-5
