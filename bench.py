@@ -83,7 +83,7 @@ def measure_roofline(s, steps=3):
     if os.path.exists(path):
         with open(path) as fh:
             traffic = json.load(fh).get("bi_pv_hbm_bytes_per_launch")
-    return {"kernel": "bi_pv<LduOp<double>> (BiCGSTAB p-update fused with the face-operator SpMV)",
+    return {"kernel": "bi_pv<LduSellOp<double>> (BiCGSTAB p-update fused with the face-operator SpMV, 64-row slice layout)",
             "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": nbytes,
             "us_per_launch": round(t * 1e6, 2), "launches_timed": int(cnt.value)}

@@ -553,6 +553,9 @@ template int bicgstab_run<CsrOp<double>, double>(int64_t, CsrOp<double>, const d
                                                  double, int64_t, int, void*, int64_t*, double*, hipStream_t, bool);
 template int bicgstab_run<LduOp<double>, double>(int64_t, LduOp<double>, const double*, const double*, double*,
                                                  double, int64_t, int, void*, int64_t*, double*, hipStream_t, bool);
+template int bicgstab_run<LduSellOp<double>, double>(int64_t, LduSellOp<double>, const double*, const double*, double*,
+                                                     double, int64_t, int, void*, int64_t*, double*, hipStream_t,
+                                                     bool);
 
 }  // namespace ftmi
 
@@ -590,6 +593,20 @@ int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indice
     CsrOp<double> A{indptr, indices, data};
     return bicgstab_run<CsrOp<double>, double>(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work, iters,
                                                res, (hipStream_t)stream, false);
+}
+
+int ftmi_bicgstab_ldu_sliced(int64_t n, const double* diag, const int32_t* base, const uint8_t* deg,
+                             const int32_t* nbr, const double* off, const double* b, const double* inv_diag, double* x,
+                             double tol, int64_t maxiter, int relative_to_initial, void* work, int64_t* iters,
+                             double* res, void* stream) {
+    if (n <= 0) {
+        *iters = 0;
+        *res = 0.0;
+        return 1;
+    }
+    LduSellOp<double> A{diag, base, deg, nbr, off};
+    return bicgstab_run<LduSellOp<double>, double>(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work,
+                                                   iters, res, (hipStream_t)stream, false);
 }
 
 int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,

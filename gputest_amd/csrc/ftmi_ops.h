@@ -51,6 +51,34 @@ struct LduOp {
     }
 };
 
+// The face operator with its half-edges in 64-row slices: the k-th half-edge of row i sits at
+// base[i / 64] + 64 k + i % 64 (k < deg[i]), so the 64 lanes of a wavefront read one coalesced
+// run of nbr/off per k instead of 64 short strided rows. Rows keep their half-edge order, so
+// y_i is summed exactly as LduOp sums it.
+template <typename T>
+struct LduSellOp {
+    const T* diag;
+    const int32_t* base;
+    const uint8_t* deg;
+    const int32_t* nbr;
+    const T* off;
+    __device__ inline T row(int64_t i, const T* __restrict__ x) const {
+        T s = diag[i] * x[i];
+        const int32_t b = base[i >> 6] + (int32_t)(i & 63);
+        const int d = deg[i];
+        for (int k = 0; k < d; ++k) s += off[b + 64 * k] * x[nbr[b + 64 * k]];
+        return s;
+    }
+    template <class F>
+    __device__ inline T row_f(int64_t i, F f) const {
+        T s = diag[i] * f(i);
+        const int32_t b = base[i >> 6] + (int32_t)(i & 63);
+        const int d = deg[i];
+        for (int k = 0; k < d; ++k) s += off[b + 64 * k] * f(nbr[b + 64 * k]);
+        return s;
+    }
+};
+
 // The same operator in difference form: y_i = ex_i x_i + sum_k off_k (x_nbr(k) - x_i) with
 // ex_i = d_i + sum_k off_k (the row sum). For a nearly singular Laplacian (row sums ~ 0) the
 // rounding of FP32 coefficients then multiplies differences of x, so the error of A x scales

@@ -47,6 +47,15 @@ __global__ void __launch_bounds__(kBlock) k_cast_f32(int64_t n, const double* __
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) y[i] = (float)x[i];
 }
 
+// soff[pos[k]] = off[k]: half-edge values into the 64-row slice layout
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_slice(int64_t n, const int32_t* __restrict__ pos,
+                                                  const T* __restrict__ off, T* __restrict__ soff) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t k = lo + threadIdx.x; k < hi; k += kBlock) soff[pos[k]] = off[k];
+}
+
 // y[j] = x[idx[j]]
 __global__ void __launch_bounds__(kBlock) k_gather_f64(int64_t n, const int64_t* __restrict__ idx,
                                                        const double* __restrict__ x, double* __restrict__ y) {
@@ -219,6 +228,12 @@ int ftmi_csr_spmm(int64_t nrows, int64_t k, const int32_t* indptr, const int32_t
 int ftmi_cast_f64_f32(int64_t n, const double* x, float* y, void* stream) {
     if (n > 0) k_cast_f32<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, x, y);
     FTMI_LAUNCH_CHECK("ftmi_cast_f64_f32");
+    return 0;
+}
+
+int ftmi_ldu_slice(int64_t nhe, const int32_t* pos, const double* off, double* soff, void* stream) {
+    if (nhe > 0) k_slice<double><<<grid_for(nhe), kBlock, 0, (hipStream_t)stream>>>(nhe, pos, off, soff);
+    FTMI_LAUNCH_CHECK("ftmi_ldu_slice");
     return 0;
 }
 

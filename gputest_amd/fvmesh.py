@@ -194,6 +194,7 @@ class FVMesh:
         self.indptr, self.indices, self.slot_diag = i32(indptr), i32(indices), i32(slot_diag)
         self.src_ptr, self.src = i32(src_ptr), i32(src)
         self.he_slot = i32(he_slot)
+        self._slices(cptr, cnbr)
         # the reference's incidence adjacency for the drop-in kernels (iidx sorted = cface >> 1)
         self.iidx = i32(cface >> 1)
         self.isgn = f64(np.where(cface & 1, -1.0, 1.0))
@@ -296,6 +297,38 @@ class FVMesh:
 
     def matrix_like(self, data) -> DeviceCSR:
         return DeviceCSR(data, self.indices, self.indptr, (self.nc, self.nc))
+
+    def _slices(self, cptr, cnbr):
+        """The half-edges in 64-row slices (the layout of ``ftmi_bicgstab_ldu_sliced``): slice g
+        holds rows 64g..64g+63, the k-th half-edge of row i at sl_base[g] + 64 k + i % 64 with
+        slice width max(deg) over its rows; sl_pos maps each half-edge (cptr order) there."""
+        nc = self.nc
+        deg = np.diff(np.asarray(cptr, np.int64))
+        if deg.size and deg.max() > 255:
+            self.sl_base = None  # more than 255 faces per cell: the sliced path is not used
+            return
+        ng = (nc + 63) // 64
+        d = np.zeros(ng * 64, np.int64)
+        d[:nc] = deg
+        width = d.reshape(ng, 64).max(axis=1)
+        base = np.concatenate([[0], np.cumsum(64 * width)])
+        rows = np.repeat(np.arange(nc), deg)
+        k = np.arange(rows.size) - np.asarray(cptr, np.int64)[rows]
+        pos = base[rows // 64] + 64 * k + rows % 64
+        self.sl_size = int(base[-1])
+        snbr = np.zeros(self.sl_size, np.int64)  # padding entries point at row 0 and are never read
+        snbr[pos] = cnbr
+        i32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int32), device=self.device)  # noqa: E731
+        self.sl_base, self.sl_nbr, self.sl_pos = i32(base[:-1]), i32(snbr), i32(pos)
+        self.sl_deg = torch.as_tensor(deg.astype(np.uint8), device=self.device)
+
+    def slice_values(self, off, out=None):
+        """Half-edge values (cptr order) in the slice layout (padding entries are never read)."""
+        if out is None:
+            out = torch.zeros(self.sl_size, dtype=torch.float64, device=self.device)
+        nat.check(nat.lib().ftmi_ldu_slice(off.numel(), nat.ptr(self.sl_pos), nat.ptr(off), nat.ptr(out),
+                                           nat.stream_ptr()), "ldu_slice")
+        return out
 
     # ------------------------------------------------------------------ host helpers
     def ldu_to_csr_host(self, diag: np.ndarray, off: np.ndarray) -> sp.csr_matrix:

@@ -74,10 +74,17 @@ def bicgstab_jacobi(A, b, x0, inv_diag, tol: float, maxiter: int, relative_to_in
     return x, info
 
 
-def bicgstab_ldu(fv, diag, off, b, inv_diag, x, tol: float, maxiter: int, relative_to_initial=True, ws=None):
-    """In place on ``x`` (x0 in, solution out); the operator is (diag, off) on ``fv``'s adjacency."""
+def bicgstab_ldu(fv, diag, off, b, inv_diag, x, tol: float, maxiter: int, relative_to_initial=True, ws=None,
+                 sliced_off=None):
+    """In place on ``x`` (x0 in, solution out); the operator is (diag, off) on ``fv``'s adjacency.
+    With ``sliced_off`` (``off`` already in ``fv``'s slice layout, ``fv.slice_values``) the solve
+    reads the sliced operator: the same sums, coalesced loads."""
     n = b.shape[0]
     work = (ws or _ws(b.device)).get("bicgstab", n)
+    if sliced_off is not None:
+        return _run(nat.lib().ftmi_bicgstab_ldu_sliced, n, nat.ptr(diag), nat.ptr(fv.sl_base), nat.ptr(fv.sl_deg),
+                    nat.ptr(fv.sl_nbr), nat.ptr(sliced_off), nat.ptr(b), nat.ptr(inv_diag), nat.ptr(x), float(tol),
+                    int(maxiter), int(bool(relative_to_initial)), nat.ptr(work))
     return _run(nat.lib().ftmi_bicgstab_ldu, n, nat.ptr(diag), nat.ptr(fv.cptr), nat.ptr(fv.cnbr), nat.ptr(off),
                 nat.ptr(b), nat.ptr(inv_diag), nat.ptr(x), float(tol), int(maxiter), int(bool(relative_to_initial)),
                 nat.ptr(work))

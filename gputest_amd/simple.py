@@ -191,6 +191,8 @@ class SimpleSolver:
         self.gpp = z(nc, dim)
         # the iteration-start fields, kept for the diagnostics of the latest iteration
         self.U_old, self.p_old = z(dim, nc), z(nc)
+        # the transport operator in the slice layout (FP64 BiCGSTAB), shared by momentum and energy
+        self.soff = z(self.fv.sl_size) if self.fv.sl_base is not None and not self.mixed else None
         if self.energy:
             self.Tb, self.gT, self.Tnew, self.bT = z(nb), z(nc, dim), z(nc), z(nc)
             self.diagT, self.diagT_r, self.offT, self.T_old = z(nc), z(nc), z(2 * nf), z(nc)
@@ -312,13 +314,15 @@ class SimpleSolver:
             self._fp32_outputs(a, dim)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "momentum assembly")
         infos = []
+        if self.soff is not None:
+            fv.slice_values(self.off, self.soff)
         for k in range(dim):
             if self.mixed:
                 infos.append(bicgstab_ldu_mixed(fv, self.ex32, self.off32, self.b32[k], self.inv_r32,
                                                 self.Ustar[k], 0.1, 30, True, ws=self.ws))
             else:
                 infos.append(bicgstab_ldu(fv, self.diag_r, self.off, self.bmom[k], self.inv_r, self.Ustar[k], 0.1,
-                                          30, True, ws=self.ws))
+                                          30, True, ws=self.ws, sliced_off=self.soff))
         self._mark("momentum")
         # 5. Rhie-Chow fluxes and the pressure-correction system
         nat.check(L.ftmi_rhie_chow(M, B, float(rho), nat.ptr(self.Ustar), nat.ptr(self.p), nat.ptr(self.gp),
@@ -438,8 +442,10 @@ class SimpleSolver:
             info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
                                       True, ws=self.ws)
         else:
+            if self.soff is not None:
+                self.fv.slice_values(self.offT, self.soff)
             info = bicgstab_ldu(self.fv, self.diagT_r, self.offT, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
-                                ws=self.ws)
+                                ws=self.ws, sliced_off=self.soff)
         self.T.copy_(self.Tnew)  # a copy, not a swap: recorded graphs keep their pointers
         return info
 

@@ -99,6 +99,14 @@ int ftmi_timer_read(double* total_ms, int64_t* launches);
 /* ---------------------------------------------------------------- Krylov on the LDU form */
 
 /* new (the reference's momentum/energy solve, simple.py:489-497, on the face layout) */
+/* new: BiCGSTAB on the face operator with its half-edges in 64-row slices (k-th half-edge of row
+ * i at base[i / 64] + 64 k + i % 64, k < deg[i]); same algorithm and sums as ftmi_bicgstab_ldu */
+int ftmi_bicgstab_ldu_sliced(int64_t n, const double* diag, const int32_t* base, const uint8_t* deg,
+                             const int32_t* nbr, const double* off, const double* b, const double* inv_diag, double* x,
+                             double tol, int64_t maxiter, int relative_to_initial, void* work, int64_t* iters,
+                             double* res, void* stream);
+/* new: soff[pos[k]] = off[k], half-edge values into the slice layout */
+int ftmi_ldu_slice(int64_t nhe, const int32_t* pos, const double* off, double* soff, void* stream);
 int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
                       const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
                       int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
