@@ -24,6 +24,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "../../include/ftmi.h"
 #include "ftmi_common.h"
 #include "ftmi_ops.h"
@@ -134,8 +136,9 @@ static int wait_state(Pinned* pn, int slot, hipStream_t st) {
 
 // ------------------------------------------------------------------ opt-in kernel timer
 // When enabled, the launches of one kernel (the BiCGSTAB p-update + SpMV, the solver's
-// largest share of time) are bracketed by HIP events on their own stream, so a benchmark
-// can read that kernel's average duration inside a real solve.
+// largest share of time) carry a start and a stop event in their dispatch
+// (hipExtLaunchKernelGGL), so a benchmark can read that kernel's average duration inside a
+// real solve, as a kernel trace would.
 struct KTimer {
     bool on = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
@@ -508,15 +511,17 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
         hipEvent_t t0 = nullptr, t1 = nullptr;
         if (g_timer.on) {
+            // events carried by the dispatch itself: the kernel's own start and end, without
+            // the dispatch latency that separate event records around the launch would add
             t0 = g_timer.get();
             t1 = g_timer.get();
-            hipEventRecord(t0, st);
-        }
-        bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
-                                           W.part[0]);
-        if (g_timer.on) {
-            hipEventRecord(t1, st);
+            hipExtLaunchKernelGGL(bi_pv<Op, T>, dim3(g), dim3(kBlock), 0, st, t0, t1, 0, n, A, (const T*)W.r,
+                                  (const T*)W.p[a], (const T*)W.v[a], inv, (const T*)W.rh, W.p[c], W.ph, W.v[c],
+                                  (const State*)W.S, W.part[0]);
             g_timer.pending.emplace_back(t0, t1);
+        } else {
+            bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
+                                               W.part[0]);
         }
         bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
                                            (int)k);
