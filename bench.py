@@ -89,6 +89,43 @@ def measure_roofline(s, steps=3):
             "us_per_launch": round(t * 1e6, 2), "launches_timed": int(cnt.value)}
 
 
+def pcg_iteration_bytes(s) -> int:
+    """Compulsory HBM bytes of one AMG-PCG iteration of the pressure solve, from the solver's
+    live hierarchy (FP64 values, int32 indices): pcg_ap (face operator + z, p_prev in, p, q out),
+    pcg_xr (p, q, x, r in; x, r out), pcg_z (r, z), and one V(2,2) cycle. Per level of n rows
+    and nnz(A): two fused pre-smoothing sweeps and the residual read A, D^-1 / b / x and write
+    one vector each; restriction reads R and r and writes the coarse b; prolongation reads P,
+    the coarse x and x and writes x; the two post-smoothing sweeps read A, D^-1, b, x and
+    write y. The dense coarse solve reads its inverse."""
+    n, nf = s.nc, s.fv.n_if
+    b = 8 * n + 4 * (n + 1) + 24 * nf + 32 * n  # pcg_ap
+    b += 48 * n + 16 * n  # pcg_xr, pcg_z
+    amg = s._amg
+    for A, P, R, _ in amg.levels:
+        nl, nc = A.n, P.ncols
+        a_bytes = 12 * A.nnz + 4 * (nl + 1)
+        b += 2 * (a_bytes + 24 * nl)  # pre2, resid
+        b += 12 * R.nnz + 4 * (nc + 1) + 8 * nl + 8 * nc  # restriction
+        b += 12 * P.nnz + 4 * (nl + 1) + 8 * nc + 16 * nl  # prolongation
+        b += 2 * (a_bytes + 32 * nl)  # post-smoothing
+    m = amg.coarse_n
+    b += 8 * m * m + 16 * m
+    return int(b)
+
+
+def pressure_solve_roofline(s, info) -> dict:
+    """Bandwidth of the pressure solve over the timed steps: PCG time (the pressure stage
+    without the AMG set-up) per PCG iteration against the algorithmic bytes of one iteration."""
+    its = info["pressure_iters_total"]
+    t = (info["stage_seconds"]["pressure"] - info["amg_setup_seconds"]) / max(its, 1)
+    nbytes = pcg_iteration_bytes(s)
+    gbs = nbytes / t / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_pcg_iteration": nbytes,
+            "us_per_pcg_iteration": round(t * 1e6, 2), "pcg_iterations": its,
+            "note": "includes each solve's start-up and the host polls; the AMG set-up is excluded"}
+
+
 def calibration_copy(nbytes=256 << 20):
     """One device-to-device copy of known size: the reference dispatch against which the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run are scaled."""
@@ -265,6 +302,7 @@ def main():
         mixed = {"value": round(args.steps / dtm, 3), "ms_per_step": round(dtm / args.steps * 1e3, 3),
                  "speedup_vs_fp64": round(dt / dtm, 3), "dtype": "f32 solves + f64 refinement", **infom}
         del sm
+    pres = pressure_solve_roofline(s, info)
     roof = measure_roofline(s)
     calibration_copy()
     out = {"metric": "SIMPLEC outer it/s at 1M cells (FP64)", "value": round(args.steps / dt, 3), "unit": "it/s",
@@ -275,7 +313,7 @@ def main():
                                   f"FP64 SIMPLEC + energy, SA-AMG-PCG tol 0.1",
                       "cells": s.nc, "interior_faces": s.fv.n_if, "precision": "fp64",
                       **info},
-           "roofline": roof}
+           "roofline": roof, "pressure_solve": pres}
     if mixed is not None:
         out["mixed"] = mixed
     if args.batch:

@@ -104,8 +104,9 @@ class SmoothedAggregationAMG:
     debug = False
 
     def __init__(self, A, device="cuda", theta=0.08, max_coarse=300, max_levels=20, sweeps=2, omega=0.8,
-                 singular=False, host_pattern=None):
+                 singular=False, host_pattern=None, refreshable=False):
         self.device = torch.device(device)
+        self.refreshable = refreshable  # keep what ``refresh`` needs (the transpose map of each P)
         self.theta, self.max_coarse, self.max_levels = theta, max_coarse, max_levels
         self.omega, self.sweeps, self.singular = omega, sweeps, singular
         A0 = A if isinstance(A, DevCSR) else DevCSR.from_scipy(A, self.device)
@@ -245,7 +246,7 @@ class SmoothedAggregationAMG:
             P = self._spgemm(A, T, xscale=dinv, agg=agg, tval=t_val, omega_num=rho, omega_scale=4.0 / 3.0,
                              keep=plan["kP"])
             R = self._transpose(P)
-            plan["perm"] = self._transpose_perm(P, R)
+            plan["perm"] = self._transpose_perm(P, R) if self.refreshable else None
             AP = self._spgemm(A, P, keep=plan["kAP"])
             Ac = self._spgemm(R, AP, keep=plan["kRAP"])
             plan["AP"] = AP
@@ -294,6 +295,8 @@ class SmoothedAggregationAMG:
         No aggregation and no host read except the small coarse matrix. The result is a valid
         smoothed-aggregation preconditioner of the new matrix built on the old aggregates (not
         the reference's hierarchy, which re-aggregates)."""
+        if not self.refreshable:
+            raise RuntimeError("this hierarchy was set up without refreshable=True")
         L, st = nat.lib(), nat.stream_ptr()
         for (A, P, R, dinv), plan in zip(self.levels, self._plans):
             n = A.n

@@ -346,7 +346,8 @@ class SimpleSolver:
                 self._amg_refreshed += 1
                 self.amg_refreshes += 1
             else:
-                self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p)
+                self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p,
+                                                           refreshable=self.mixed)
                 self._amg32 = self._amg.to_fp32() if self.mixed else None
                 self._amg_refreshed = 0
                 for k, v in self._amg.timing.items():

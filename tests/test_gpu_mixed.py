@@ -169,7 +169,7 @@ def test_numeric_refresh_matches_the_hierarchy_on_the_kept_aggregates():
     fv = FVMesh(cartesian_mesh_2d(48, 40, triangulate=True, warp=0.05), DEV)
     A1, d1, off1 = _ldu_system(fv, 7, symmetric=True)
     A2, d2, off2 = _ldu_system(fv, 8, symmetric=True)
-    amg = SmoothedAggregationAMG.from_ldu(fv, t(d1), t(off1))
+    amg = SmoothedAggregationAMG.from_ldu(fv, t(d1), t(off1), refreshable=True)
     amg32 = amg.to_fp32()
     amg.refresh_ldu(fv, t(d2), t(off2))
     amg32.refresh()
