@@ -778,7 +778,7 @@ int ftmi_amgs_spgemm(int fill, int64_t nrows, int64_t ncols, const int32_t* x_pt
 // out[0..n] = exclusive prefix sums of in[0..n-1] (out[n] = total); bsum: >= 2048 int64.
 int ftmi_scan_i32(int64_t n, const int32_t* in, int32_t* out, int64_t* bsum, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    const int g = grid_for(n > 0 ? n : 1);
+    const int g = grid_part(n > 0 ? n : 1);
     k_scan_blocks<<<g, kBlock, 0, s>>>(n, in, bsum);
     k_scan_apply<<<g, kBlock, 0, s>>>(n, in, bsum, out);
     FTMI_LAUNCH_CHECK("ftmi_scan_i32");

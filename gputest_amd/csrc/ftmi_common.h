@@ -31,7 +31,7 @@ struct RedBuf {
 };
 
 // Workgroup cap of the streaming kernels (kMaxGrid by default; the autotuner may change it
-// through ftmi_set_grid_cap). Reductions stay within kRedGrid partials whatever the cap.
+// through ftmi_set_grid_cap, up to 65536).
 int grid_cap();
 
 inline int grid_for(int64_t n) {
@@ -40,6 +40,14 @@ inline int grid_for(int64_t n) {
     const int cap = grid_cap();
     if (g > cap) g = cap;
     return (int)g;
+}
+
+// Grid of a kernel that writes per-workgroup values into a caller's buffer sized for kMaxGrid
+// workgroups (RedBuf partials: kMaxGrid * kMaxRed doubles; scan block sums: kMaxGrid): the
+// streaming grid, but never more than kMaxGrid workgroups whatever the cap.
+inline int grid_part(int64_t n) {
+    const int g = grid_for(n);
+    return g < kMaxGrid ? g : kMaxGrid;
 }
 
 // Contiguous row range [lo, hi) of this workgroup with the XCD-aware block permutation.

@@ -756,7 +756,7 @@ int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi
     A.resid = a->resid;
     for (int q = 0; q < 3; ++q) A.x0[q] = q < nfld ? a->x0[q] : nullptr;
     hipStream_t s = (hipStream_t)stream;
-    const int g = grid_for(m.nc);
+    const int g = grid_part(m.nc);
     RedBuf rb = red(rbuf);
     if (a->energy ? nfld != 1 : nfld != mesh->dim) {
         set_error_msg(a->energy ? "energy assembly takes one field" : "momentum assembly takes dim fields");
@@ -782,7 +782,7 @@ int ftmi_rhie_chow(const ftmi_mesh* mesh, const ftmi_bc* bc, double rho, const d
                    double* diag_p, double* offp, double* mass, double* neg_mass, double* cont_out,
                    const ftmi_red* rbuf, void* stream) {
     MeshV m = view(mesh);
-    const int g = grid_for(m.nc);
+    const int g = grid_part(m.nc);
     DISPATCH_DIM(mesh->dim, (k_rhie_chow<D><<<g, kBlock, 0, (hipStream_t)stream>>>(
                                 m, bview(bc), rho, Ustar, p, gp, dP, Fst, cf, Fbst, cb, diag_p, offp, mass, neg_mass,
                                 cont_out, red(rbuf))));
@@ -797,7 +797,7 @@ int ftmi_correct(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* pp, con
                  const ftmi_red* rbuf, void* stream) {
     MeshV m = view(mesh);
     hipStream_t s = (hipStream_t)stream;
-    const int g = grid_for(m.nc);
+    const int g = grid_part(m.nc);
     DISPATCH_DIM(mesh->dim, (k_correct<D><<<g, kBlock, 0, s>>>(m, bview(bc), pp, Ustar, dP, Fst, cf, Fbst, cb, relax_p,
                                                                 U, p, F, Fb, gpp_out, sums_out, red(rbuf))));
     fin_store<3><<<1, kBlock, 0, s>>>(rbuf->partials, g, sums_out, 1.0);
@@ -816,8 +816,8 @@ int ftmi_init_flux(const ftmi_mesh* mesh, const ftmi_bc* bc, double rho, const d
 }
 
 int ftmi_abs_sum(int64_t n, const double* x, double* out, const ftmi_red* rbuf, void* stream) {
-    k_abs_sum<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, x, out, red(rbuf));
-    fin_store<1><<<1, kBlock, 0, (hipStream_t)stream>>>(rbuf->partials, grid_for(n), out, 1.0);
+    k_abs_sum<<<grid_part(n), kBlock, 0, (hipStream_t)stream>>>(n, x, out, red(rbuf));
+    fin_store<1><<<1, kBlock, 0, (hipStream_t)stream>>>(rbuf->partials, grid_part(n), out, 1.0);
     FTMI_LAUNCH_CHECK("ftmi_abs_sum");
     return 0;
 }
