@@ -23,8 +23,9 @@ Restated from (file:line in the reference):
 
 Scope: laminar flow, optional energy, walls / velocity inlets (uniform or per-face
 profile) / pressure outlets / pressure openings / symmetry / translational periodic
-pairs, body force and periodic mean pressure gradient, SIMPLE and SIMPLEC, steady.
-Turbulence, MRF, radiation, species and time stepping are outside the hot path.
+pairs, body force and periodic mean pressure gradient, SIMPLE and SIMPLEC, steady; the
+k-omega SST model of ``oracle/turbulence_oracle.py`` when ``Case.turbulence`` names it.
+MRF, radiation, species and time stepping are outside the hot path.
 
 Pinning: every function here is checked against the golden vectors written by
 ``tests/golden/make_golden.py`` from the reference itself (captured intermediate
@@ -75,7 +76,7 @@ class FV:
 
         self.mesh = mesh
         dim = self.dim = mesh.dim
-        geo = geometry if geometry is not None else compute_geometry(mesh)
+        geo = self.geo = geometry if geometry is not None else compute_geometry(mesh)
         nc = self.nc = mesh.n_cells
         inner = mesh.neighbour >= 0
         bnd = ~inner
@@ -426,6 +427,11 @@ class Boundary:
     thermal: str = "adiabatic"
     value: float = 0.0
     profile: Optional[Callable] = None  # xfb (n, dim) -> (n, dim) fixed velocity
+    # turbulence entering at this zone (None: the case defaults) (case.py:53-58)
+    turb_intensity: Optional[float] = None
+    turb_viscosity_ratio: Optional[float] = None
+    k: Optional[float] = None
+    omega: Optional[float] = None
 
 
 @dataclass
@@ -450,9 +456,22 @@ class Case:
     tol: float = 1e-5
     init_velocity: tuple = (0.0, 0.0, 0.0)
     init_T: float = 0.0
+    # turbulence (case.py:99-110): "laminar" or "k-omega-sst"
+    turbulence: str = "laminar"
+    turb_intensity: float = 0.05
+    turb_viscosity_ratio: float = 10.0
+    turb_reference_velocity: Optional[float] = None
+    turb_init: Optional[dict] = None
+    relax_k: Optional[float] = None
+    relax_w: Optional[float] = None
+    turb_scheme: str = "upwind"
 
     def __post_init__(self):
         self.boundaries = dict(self.boundaries or {})
+        self.turbulence = {"sst": "k-omega-sst", "kw-sst": "k-omega-sst", "k-omega": "k-omega-sst",
+                           "none": "laminar"}.get(str(self.turbulence).lower(), str(self.turbulence).lower())
+        if self.turbulence not in ("laminar", "k-omega-sst"):
+            raise ValueError(f"turbulence {self.turbulence!r}: the oracle restates laminar and k-omega-sst")
         self.algorithm = self.algorithm.upper()
         if self.relax_p is None and self.algorithm == "SIMPLE":
             self.relax_p = 0.3
@@ -485,6 +504,11 @@ class Simplec:
         self.T = np.full(self.nc, case.init_T) if self.energy else None
         self.F = case.density * np.sum(fv.face_interp_vec(self.U) * fv.S, axis=1)
         self.Fb = self._fixed_flux(self._ubvals(self.U))
+        self.turb = None
+        if case.turbulence == "k-omega-sst":
+            from oracle.turbulence_oracle import SST
+
+            self.turb = SST(self)
         self.guard_enabled = case.relax_p is None
         self._capture = None
         self._reset_guard()
@@ -494,12 +518,14 @@ class Simplec:
         fv, dim, nb = self.fv, self.dim, self.fv.n_bf
         kind = np.zeros(nb, np.int64)
         ub, pb, tk, tv = np.zeros((nb, dim)), np.zeros(nb), np.zeros(nb, np.int64), np.zeros(nb)
+        self.zone_specs = {}
         for z in self.mesh.boundary_zones():
             spec = self.case.boundaries.get(z.name)
             if spec is None:
                 spec = Boundary(kind=_FLUENT_KIND.get(z.type, "wall"))
                 if spec.kind == "inlet":
                     spec.normal_speed = 1.0
+            self.zone_specs[z.name] = spec
             m = fv.bzone == z.id
             kind[m] = _KIND[spec.kind]
             if spec.kind in ("wall", "inlet"):
@@ -545,19 +571,22 @@ class Simplec:
 
     # --- discretisation (simple.py:340-387) ---
     def _coefficients(self, gamma, dirichlet):
+        """``gamma``: a scalar, or (interior-face values, boundary-face values) (simple.py:333-353)."""
         fv, F, Fb = self.fv, self.F, self.Fb
-        D = gamma * fv.alpha
+        gf, gb = gamma if isinstance(gamma, tuple) else (gamma, gamma)
+        D = gf * fv.alpha
         aPN = -(D + np.maximum(-F, 0.0))
         aNP = -(D + np.maximum(F, 0.0))
-        Db = gamma * fv.alpha_b * dirichlet
+        Db = gb * fv.alpha_b * dirichlet
         diag = fv.scatter(D + np.maximum(F, 0.0), D + np.maximum(-F, 0.0), Db + np.maximum(Fb, 0.0))
         return diag, aPN, aNP, Db
 
-    def _rhs(self, gamma, phi, grad, phib, Db, extra=None):
+    def _rhs(self, gamma, phi, grad, phib, Db, extra=None, scheme=None):
         fv, F, Fb = self.fv, self.F, self.Fb
+        gf = gamma[0] if isinstance(gamma, tuple) else gamma
         rhs = fv.Cb @ ((Db + np.maximum(-Fb, 0.0)) * phib)
-        rhs = rhs + fv.Cdiff @ (gamma * np.sum(fv.face_interp_vec(grad) * fv.Tvec, axis=1))
-        if self.case.scheme == "second-order":
+        rhs = rhs + fv.Cdiff @ (gf * np.sum(fv.face_interp_vec(grad) * fv.Tvec, axis=1))
+        if (scheme or self.case.scheme) == "second-order":
             phP, phN = phi[fv.P], phi[fv.N]
             hoP = phP + np.sum(grad[fv.P] * fv.dP_f, axis=1)
             hoN = phN + np.sum(grad[fv.N] * fv.dN_f, axis=1)
@@ -597,10 +626,14 @@ class Simplec:
                                                                      + [0.0] * 3)[:dim], float)
         if np.any(drive):
             src = src + fv.vol[:, None] * drive[None, :]
+        gm = mu
+        if self.turb is not None:  # mu + mu_t on faces, the wall-function viscosity on walls
+            gm = self.turb.momentum_viscosity(U)
+            src = src + self.turb.transpose_stress(gU)
         res = {}
         au = self.relax_u
-        diag, aPN, aNP, Db = self._coefficients(mu, self.m_dirvel)
-        rhs = [self._rhs(mu, U[:, k], gU[k], ub[:, k], Db, src[:, k]) for k in range(dim)]
+        diag, aPN, aNP, Db = self._coefficients(gm, self.m_dirvel)
+        rhs = [self._rhs(gm, U[:, k], gU[k], ub[:, k], Db, src[:, k]) for k in range(dim)]
         A = fv.matrix(diag, aPN, aNP)
         num = [float(np.sum(np.abs(rhs[k] - A @ U[:, k]))) for k in range(dim)]
         den = [float(np.sum(np.abs(diag * U[:, k]))) for k in range(dim)]
@@ -657,6 +690,13 @@ class Simplec:
                          "p_new": self.p, "alpha_u": au, "mom_info": infos, "p_info": pinfo}
         if self.energy:
             res["energy"] = self._energy_step()
+        if self.turb is not None:  # simple.py:662-669: gradients of the corrected velocity
+            ubc = self._ubvals(self.U, gU)
+            gUc = [fv.gradient(self.U[:, k], ubc[:, k]) for k in range(dim)]
+            k_old, w_old = self.turb.k, self.turb.w
+            res.update(self.turb.solve(gUc))
+            self._capture["turbulence"] = {"k": (k_old, self.turb.k), "omega": (w_old, self.turb.w),
+                                           "gU": gUc}
         self.iteration += 1
         self.last_linear = {"momentum_iters": its, "pressure_iters": pinfo.iterations,
                             "pressure_residual": pinfo.residual}
@@ -679,6 +719,8 @@ class Simplec:
     def _energy_step(self):
         c, fv = self.case, self.fv
         gam = c.conductivity / c.cp
+        if self.turb is not None:
+            gam = self.turb.energy_diffusivity(gam)  # k/cp + mu_t/Pr_t
         T = self.T
         TP = T[fv.Pb]
         fixed = self.tkind == T_FIXED
@@ -762,6 +804,8 @@ class Simplec:
               "res_continuity_norm": self._res_max5.get("continuity", 0.0)}
         if self.energy:
             st["T"] = self.T.copy()
+        if self.turb is not None:
+            st.update(self.turb.state())
         return st
 
     def load_state(self, st):
@@ -771,6 +815,8 @@ class Simplec:
             self.T = np.array(st["T"], float)
         self.iteration = int(st.get("iteration", 0))
         self._gp = self._gU = self._gT = None
+        if self.turb is not None:
+            self.turb.load(st)
         if float(st.get("res_continuity_norm", 0.0)) > 0:
             self._res_max5["continuity"] = float(st["res_continuity_norm"])
 
