@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftmi.so")
-ABI_VERSION = 101
+ABI_VERSION = 102
 
 P = ctypes.c_void_p
 I32, I64, F64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -41,11 +41,26 @@ class FtmiAssembleArgs(ctypes.Structure):
                 ("simplec", I32), ("cp", F64),
                 ("diag", P), ("diag_r", P), ("off", P), ("b", P * 3), ("dP", P), ("resid", P),
                 ("inv_r", P), ("x0", P * 3), ("out_fp32", I32),
-                ("diag32", P), ("diag_r32", P), ("off32", P), ("b32", P * 3), ("inv_r32", P), ("ex32", P)]
+                ("diag32", P), ("diag_r32", P), ("off32", P), ("b32", P * 3), ("inv_r32", P), ("ex32", P),
+                ("gam_f", P), ("gam_b", P), ("extra", P * 3)]
+
+
+class FtmiSstCoef(ctypes.Structure):
+    _fields_ = [("rho", F64), ("mu", F64), ("two_rho_sw2", F64), ("four_rho_sw2", F64), ("nu500", F64),
+                ("ten_bstar_rho", F64), ("bstar_rho", F64), ("rho_a1", F64), ("g1", F64), ("g2", F64)]
 
 
 _SIGS = {
     "ftmi_abi_version": ([], I32),
+    "ftmi_sst_bvals": ([P] * 10 + [P], I32),
+    "ftmi_sst_cell": ([P] * 17, I32),
+    "ftmi_sst_work_doubles": ([], I32),
+    "ftmi_sst_assemble": ([P] * 11 + [F64, F64] + [P] * 10, I32),
+    "ftmi_sst_hold": ([I64] + [P] * 5 + [F64, F64] + [P] * 6, I32),
+    "ftmi_sst_update": ([I64, P, P, F64, F64] + [P] * 7, I32),
+    "ftmi_sst_face_gamma": ([P, P, P, F64, I32, P, P, P], I32),
+    "ftmi_sst_wall": ([P, P, P, P, F64, F64, P, P, P, P], I32),
+    "ftmi_sst_tstress": ([P] * 5, I32),
     "ftmi_last_error": ([], ctypes.c_char_p),
     "ftmi_device_count": ([], I32),
     "ftmi_csr_spmm": ([I64, I64, P, P, P, P, P, P], I32),

@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libftmi.so")
-HIP_SOURCES = ["ftmi_ops.hip", "ftmi_krylov.hip", "ftmi_simplec.hip", "ftmi_amg_setup.hip", "ftmi_diag.hip"]
+HIP_SOURCES = ["ftmi_ops.hip", "ftmi_krylov.hip", "ftmi_simplec.hip", "ftmi_amg_setup.hip", "ftmi_diag.hip",
+               "ftmi_turb.hip"]
 CXX_SOURCES = ["ftmi_host.cpp"]
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

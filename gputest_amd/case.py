@@ -1,7 +1,8 @@
 """Case description of the SIMPLEC hot path: the reference's ``UnstructuredCase`` and
 ``BoundarySpec`` (``flowtrace/unstructured/case.py:31-177``), restricted to what the hot
-path uses (laminar, optional energy, steady), plus one addition: ``BoundarySpec.profile``,
-a per-face fixed velocity for walls and inlets (the workloads' parabolic inlet)."""
+path uses (laminar or k-omega SST, optional energy, steady), plus one addition:
+``BoundarySpec.profile``, a per-face fixed velocity for walls and inlets (the workloads'
+parabolic inlet)."""
 from __future__ import annotations
 
 from dataclasses import asdict, dataclass, field
@@ -14,6 +15,8 @@ FLUENT_TO_KIND = {
     "axis": "symmetry", "periodic": "periodic", "periodic-shadow": "periodic",
 }
 KINDS = ("wall", "inlet", "outlet", "pressure", "symmetry", "periodic")
+TURBULENCE_ALIASES = {"sst": "k-omega-sst", "kw-sst": "k-omega-sst", "k-omega": "k-omega-sst", "none": "laminar"}
+TURBULENCE_MODELS = ("laminar", "k-omega-sst")
 
 
 @dataclass
@@ -34,6 +37,11 @@ class BoundarySpec:
     thermal: str = "adiabatic"
     value: float = 0.0
     profile: Optional[Callable] = None
+    # turbulence entering at this zone (None: the case defaults) (case.py:53-58)
+    turb_intensity: Optional[float] = None
+    turb_viscosity_ratio: Optional[float] = None
+    k: Optional[float] = None
+    omega: Optional[float] = None
 
     def __post_init__(self):
         if self.kind not in KINDS:
@@ -67,6 +75,15 @@ class UnstructuredCase:
     precision: str = "fp64"
     periodic: List[List[str]] = field(default_factory=list)
     periodic_pressure_gradient: List[float] = field(default_factory=lambda: [0.0, 0.0, 0.0])
+    # turbulence (case.py:99-110): laminar | k-omega-sst
+    turbulence: str = "laminar"
+    turb_intensity: float = 0.05
+    turb_viscosity_ratio: float = 10.0
+    turb_reference_velocity: Optional[float] = None
+    turb_init: Optional[Dict[str, float]] = None
+    relax_k: Optional[float] = None
+    relax_w: Optional[float] = None
+    turb_scheme: str = "upwind"
 
     def __post_init__(self):
         self.boundaries = {k: (BoundarySpec(**v) if isinstance(v, dict) else v) for k, v in self.boundaries.items()}
@@ -83,6 +100,16 @@ class UnstructuredCase:
             raise ValueError("precision must be 'fp64' or 'mixed' (FP32 transport assembly and Krylov solves, "
                              "FP32 AMG-PCG refined in FP64)")
         self.periodic = [list(p) for p in self.periodic]
+        t = str(self.turbulence).lower()
+        self.turbulence = TURBULENCE_ALIASES.get(t, t)
+        if self.turbulence not in TURBULENCE_MODELS:
+            raise ValueError(f"turbulence must be one of {TURBULENCE_MODELS}")
+        if self.turbulence != "laminar" and self.turb_scheme != "upwind":
+            raise ValueError("the SST transports are built for turb_scheme='upwind' (the reference's default)")
+
+    @property
+    def turbulent(self) -> bool:
+        return self.turbulence != "laminar"
 
     def to_dict(self) -> dict:
         d = asdict(self)

@@ -279,6 +279,9 @@ struct AsmArgs {
     const double* F;
     const double* Fb;
     double gamma;
+    const double* gam_f;     // per interior face diffusivity (nullptr: gamma everywhere)
+    const double* gam_b;     // per boundary face diffusivity (nullptr: gamma)
+    const double* extra[3];  // per-cell source added to field q's source (nullptr: none)
     int dir_mode;  // 0: momentum (walls, inlets, symmetry Dirichlet); 1: energy (fixed-T faces)
     const double* phi[3];
     const double* grad[3];
@@ -338,7 +341,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             const bool isN = code & 1;
             const int64_t o = m.cnbr[k];
             const R F = (R)A.F[f];
-            const R D = gamma * (R)m.alpha[f];
+            const R gf = A.gam_f ? (R)A.gam_f[f] : gamma;
+            const R D = gf * (R)m.alpha[f];
             const R aPN = -(D + fmax0(-F));
             const R aNP = -(D + fmax0(F));
             const R a_here = isN ? aNP : aPN;
@@ -363,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
 #pragma unroll
                 for (int a = 1; a < DIM; ++a)
                     gt += (wf * (R)g[P * DIM + a] + wn * (R)g[N * DIM + a]) * (R)m.Tv[f * DIM + a];
-                const R tn = gamma * gt;
+                const R tn = gf * gt;
                 R dc = R(0);
                 if (A.second_order) {
                     const R phP = isN ? po : pc[q], phN = isN ? pc[q] : po;
@@ -394,7 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
                 dir = (R)m_dirvel(kd);
             else
                 dir = bc.tkind[f] == T_FIXED ? R(1) : R(0);
-            const R Db = gamma * (R)m.alpha_b[f] * dir;
+            const R Db = (A.gam_b ? (R)A.gam_b[f] : gamma) * (R)m.alpha_b[f] * dir;
             dg_b += Db + fmax0(Fb);
             const R cb = Db + fmax0(-Fb);
 #pragma unroll
@@ -419,6 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             if (A.dir_mode == 0) {
                 R src = -(R)A.gp[c * DIM + q] * vol;
                 if (A.has_drive) src = src + vol * (R)A.drive[q];
+                if (A.extra[q]) src = src + (R)A.extra[q][c];
                 rhs = rhs + src;
             } else {
                 rhs = rhs + qin;
@@ -738,6 +743,9 @@ int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi
     A.F = a->F;
     A.Fb = a->Fb;
     A.gamma = a->gamma;
+    A.gam_f = a->gam_f;
+    A.gam_b = a->gam_b;
+    for (int q = 0; q < 3; ++q) A.extra[q] = a->extra[q];
     A.dir_mode = a->energy ? 1 : 0;
     const int nfld = a->nfields;
     for (int q = 0; q < nfld && q < 3; ++q) {

@@ -52,7 +52,8 @@ _S_MOM = 0  # 2 per component: sum |b - A u|, sum |diag u|
 _S_CONT = 6
 _S_SUMS = 7  # sum p_new, sum |F|, sum |Fb|
 _S_EN = 10  # energy: sum |b - A T|, sum |diag T|
-_NSCAL = 16
+_S_TURB = 16  # SST: k then omega, each sum |r|, sum |diag phi|, max diag
+_NSCAL = 22
 
 
 def auto_relax_p(non_orthogonality_deg: float) -> float:
@@ -122,6 +123,11 @@ class SimpleSolver:
         self._has_lag = False
         self._has_lagT = False
         self._init_flux()
+        self.turb = None
+        if case.turbulent:
+            from .turbulence import DeviceSST
+
+            self.turb = DeviceSST(self)
         self.guard_enabled = case.relax_p is None
         self._capture = None
         self._reset_guard()
@@ -235,6 +241,8 @@ class SimpleSolver:
             return 0.0
         self._events = []
         if self.use_graph:
+            if self.turb is not None:
+                raise NotImplementedError("graph replay of the SST outer iteration is not built; set use_graph=False")
             return self._graph_step()
         self._mark("start")
         out = self._launch(rebuild=True)
@@ -312,6 +320,11 @@ class SimpleSolver:
         a.resid, a.inv_r = nat.ptr(self.scal[_S_MOM:]), nat.ptr(self.inv_r)
         if self.mixed:
             self._fp32_outputs(a, dim)
+        if self.turb is not None:  # mu + mu_t, wall functions, transpose stress (simple.py:546-548)
+            gm_f, gm_b, extra = self.turb.momentum_terms(L, M, B, s)
+            a.gam_f, a.gam_b = nat.ptr(gm_f), nat.ptr(gm_b)
+            for k in range(dim):
+                a.extra[k] = nat.ptr(extra[k])
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "momentum assembly")
         infos = []
         if self.soff is not None:
@@ -373,6 +386,8 @@ class SimpleSolver:
         tinfo = None
         if self.energy:
             tinfo = self._energy_step(L, s, M, B, R)
+        if self.turb is not None:  # simple.py:662-669
+            self.turb.solve(L, M, B, s, self.scal[_S_TURB:], self.scal[_S_TURB + 3:])
         self._mark("correct+energy")
         return infos, pinfo, tinfo
 
@@ -392,6 +407,9 @@ class SimpleSolver:
         if self.energy:
             r, sd = float(sc[_S_EN]), float(sc[_S_EN + 1])
             res["energy"] = r / sd if sd > 0 else r
+        if self.turb is not None:
+            res["k"] = float(sc[_S_TURB]) / max(float(sc[_S_TURB + 1]), 1e-300)
+            res["omega"] = float(sc[_S_TURB + 3]) / max(float(sc[_S_TURB + 4]), 1e-300)
         self.iteration += 1
         self.step_count = self.iteration
         self.last_linear = {"momentum_iters": [i.iterations for i in infos], "pressure_iters": pinfo.iterations,
@@ -400,6 +418,8 @@ class SimpleSolver:
             self.last_linear["pressure_rounds"] = pinfo.rounds
         if tinfo is not None:
             self.last_linear["energy_iters"] = tinfo.iterations
+        if self.turb is not None and self.turb.last_info:
+            self.last_linear["turbulence_iters"] = [i.iterations for i in self.turb.last_info.values()]
         # what the diagnostics of this iteration need besides the device systems it left behind
         self._capture = {"mom_info": infos, "p_info": pinfo, "energy_info": tinfo, "iteration": self.iteration,
                          "alpha_u": float(self.relax_u), "alpha_T": float(c.relax_T), "rolled_back": False}
@@ -438,6 +458,9 @@ class SimpleSolver:
         a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(self.inv_r)
         if self.mixed:
             self._fp32_outputs(a, 1)
+        if self.turb is not None:  # k/cp + mu_t / Pr_t (simple.py:781-782)
+            ge_f, ge_b = self.turb.energy_terms(L, M, B, s, c.conductivity / c.cp)
+            a.gam_f, a.gam_b = nat.ptr(ge_f), nat.ptr(ge_b)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
         if self.mixed:
             info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
@@ -496,6 +519,8 @@ class SimpleSolver:
               "iteration": self.iteration, "res_continuity_norm": self._res_max5.get("continuity", 0.0)}
         if self.energy:
             st["T"] = self.T.clone()
+        if self.turb is not None:
+            st.update(self.turb.snapshot())
         return st
 
     def _restore_snapshot(self, st):
@@ -505,6 +530,8 @@ class SimpleSolver:
         self.Fb.copy_(st["Fb"])
         if self.energy and "T" in st:
             self.T.copy_(st["T"])
+        if self.turb is not None:
+            self.turb.load(st)
         self.iteration = int(st["iteration"])
         self._has_lag = self._has_lagT = False
         if float(st.get("res_continuity_norm", 0.0)) > 0:
@@ -573,6 +600,8 @@ class SimpleSolver:
         out["pressure"] = self.p.cpu().numpy()
         if self.energy:
             out["temperature"] = self.T.cpu().numpy()
+        if self.turb is not None:
+            out.update(self.turb.fields())
         if names:
             out = {k: v for k, v in out.items() if k in names}
         return out
@@ -586,6 +615,8 @@ class SimpleSolver:
               "res_continuity_norm": self._res_max5.get("continuity", 0.0)}
         if self.energy:
             st["T"] = self.T.cpu().numpy()
+        if self.turb is not None:
+            st.update(self.turb.state())
         return st
 
     def load_state(self, st: dict):
@@ -596,6 +627,8 @@ class SimpleSolver:
         self.Fb.copy_(torch.as_tensor(np.asarray(st["Fb"], float), device=dev))
         if self.energy and "T" in st:
             self.T.copy_(torch.as_tensor(np.asarray(st["T"], float), device=dev))
+        if self.turb is not None:
+            self.turb.load(st)
         self.iteration = self.step_count = int(st.get("iteration", 0))
         self._has_lag = self._has_lagT = False
         self._capture = None

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 101
+#define FTMI_ABI_VERSION 102
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -305,7 +305,67 @@ typedef struct ftmi_assemble_args {
     float *diag32, *diag_r32, *off32, *b32[3], *inv_r32;
     float* ex32;              /* out_fp32: row sums diag_r + sum off of the relaxed operator,
                                  built from the face fluxes (the difference form) */
+    /* face-varying diffusivity (simple.py:333-338, the (interior, boundary) form of gamma):
+     * gam_f (nf) and gam_b (nb), or NULL for `gamma` everywhere */
+    const double *gam_f, *gam_b;
+    const double* extra[3];   /* momentum: a further per-cell source of field q (the turbulent
+                                 transpose stress, simple.py:546-548), or NULL */
 } ftmi_assemble_args;
+
+
+/* ---------------------------------------------------------------- k-omega SST
+ * (flowtrace/unstructured/turbulence.py:116-437; new entry points, the reference computes the
+ * model in NumPy/CuPy array code). Products of constants are formed by the caller in the
+ * reference's left-to-right order so that the device arithmetic is the reference's. */
+typedef struct ftmi_sst_coef {
+    double rho, mu;
+    double two_rho_sw2;    /* 2 rho sigma_w2 */
+    double four_rho_sw2;   /* 4 rho sigma_w2 */
+    double nu500;          /* 500 nu */
+    double ten_bstar_rho;  /* 10 beta* rho */
+    double bstar_rho;      /* beta* rho */
+    double rho_a1;         /* rho a1 */
+    double g1, g2;         /* gamma_1, gamma_2 */
+} ftmi_sst_coef;
+
+/* k-omega SST: boundary values of k and omega, fixed (1) at inlets and backflow faces
+ * (turbulence.py:263-269) */
+int ftmi_sst_bvals(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* Fb, const double* k, const double* w,
+                   const double* k_in, const double* w_in, double* kb, double* wb, double* fixed, void* stream);
+/* strain rate S, and per cell the diffusivities and the volume-weighted sources of the k and
+ * omega transports from gU (dim, nc, dim) and the LS gradients gk, gw (nc, dim)
+ * (turbulence.py:99-113, 367-380, 419-433) */
+int ftmi_sst_cell(const ftmi_mesh* mesh, const ftmi_sst_coef* coef, const double* gU, const double* gk,
+                  const double* gw, const double* k, const double* w, const double* mut, const double* y,
+                  double* S, double* gam_k, double* sck, double* spk, double* gam_w, double* scw, double* spw,
+                  void* stream);
+/* doubles of `work` that ftmi_sst_assemble needs */
+int ftmi_sst_work_doubles(void);
+/* one SST transport system (turbulence.py:271-304): LDU coefficients (off in the cell-centric
+ * half-edge order), diag, rhs, the relaxed diag_r / b / inv_r, x0 = phi; out3 (device): sum
+ * |rhs - A phi| and sum |diag phi| over the cells not at the floor and not `held`, max diag */
+int ftmi_sst_assemble(const ftmi_mesh* mesh, const double* F, const double* Fb, const double* gam, const double* phi,
+                      const double* grad, const double* phib, const double* fixed, const double* Sc, const double* Sp,
+                      const uint8_t* held, double floor_v, double relax, double* diag, double* rhs, double* diag_r,
+                      double* off, double* b, double* inv_r, double* x0, double* work, double* out3, void* stream);
+/* the wall cells' omega held by the penalty 1e10 max(diag) (turbulence.py:286-290, 390-397);
+ * k is the field before the update, out3 the assembly's */
+int ftmi_sst_hold(int64_t nh, const int32_t* cells, const double* y_wall_cell, const double* k, const double* phi,
+                  const double* out3, double nu, double relax, const double* diag, const double* rhs, double* diag_r,
+                  double* b, double* inv_r, void* stream);
+/* floors, then mu_t = rho a1 k / max(a1 omega, S F2) of the new fields (turbulence.py:382-388) */
+int ftmi_sst_update(int64_t n, const double* xk, const double* xw, double kmin, double wmin, const double* y,
+                    const double* S, const ftmi_sst_coef* coef, double* k, double* w, double* mut, void* stream);
+/* face diffusivities: mode 0 momentum (c0 = mu; mu + mu_t), mode 1 energy (c0 = k/cp; +
+ * mu_t/Pr_t, none on walls) (turbulence.py:235-249) */
+int ftmi_sst_face_gamma(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* mut, double c0, int mode,
+                        double* gam_f, double* gam_b, void* stream);
+/* wall-function viscosity on wall faces of gam_b and y+ (nb) from Spalding's law
+ * (turbulence.py:65-87, 399-417); iters: one device int of work */
+int ftmi_sst_wall(const ftmi_mesh* mesh, const ftmi_bc* bc, const double* U, const double* yb, double mu, double nu,
+                  int* iters, double* gam_b, double* yplus, void* stream);
+/* explicit div(mu_t (grad U)^T) per cell, out (dim, nc) (turbulence.py:251-261) */
+int ftmi_sst_tstress(const ftmi_mesh* mesh, const double* mut, const double* gU, double* out, void* stream);
 
 /* new: boundary velocities and pressures (simple.py:305-325); U is (dim, nc) component-major,
  * gU is (dim, nc, dim), ub is (dim, nb); gU/gp may be NULL (no lagged reconstruction) */
