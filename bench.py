@@ -205,6 +205,35 @@ def adjoint_leg(n_body, tol, max_iter, gmres_tol):
             "Cd": round(2.0 * J, 6), "dCd_dnu": round(2.0 * float(gnu), 6), "parameters": 65}
 
 
+def sst_leg(nx, ny, steps, warmup):
+    """k-omega SST on the reference's turbulence validation case scaled to nx x ny cells
+    (flowtrace/validation/turbulent.py:47-75): a half channel of length 4, wall at y = 0 and
+    symmetry at y = 1, quads graded geometrically from a first cell of 0.065 / ny (ny <= 300:
+    the grading's root bracket overflows beyond), periodic in x, driven by dp/dx = -1 at
+    Re_tau = 392.24, started from U = 15; FP64."""
+    import numpy as np
+
+    from gputest_amd import case as C
+    from gputest_amd.mesh import cartesian_mesh_2d, geometric, make_periodic
+    from gputest_amd.simple import SimpleSolver
+
+    ys = geometric(ny, 1.0, 0.065 / ny)
+    mesh = cartesian_mesh_2d(nx, ny, xs=np.linspace(0.0, 4.0, nx + 1), ys=ys, names=("in", "out", "wall", "centre"),
+                             types=("wall", "wall", "wall", "symmetry"))
+    mesh = make_periodic(mesh, "in", "out")
+    B = C.BoundarySpec
+    case = C.UnstructuredCase(boundaries={"wall": B("wall"), "centre": B("symmetry")}, viscosity=1.0 / 392.24,
+                              body_force=[1.0, 0.0, 0.0], turbulence="k-omega-sst", turb_reference_velocity=15.0,
+                              init_velocity=[15.0, 0.0, 0.0], relax_u=0.9, pressure_tol=0.1, max_iter=10 ** 9,
+                              tol=0.0)
+    s, dt, info = run_timed(SimpleSolver, mesh, case, warmup, steps, "sst")
+    y = s.turb.fields()["y-plus"]
+    return {"workload": f"SST half channel Re_tau 392, {nx}x{ny} graded quads, periodic", "cells": s.nc,
+            "steps": steps, "it_per_s": round(steps / dt, 3), "ms_per_step": round(dt / steps * 1e3, 2),
+            "precision": "fp64", "max_y_plus": round(float(y.max()), 3),
+            "residuals": s.residuals()["scaled"], "last_linear": s.last_linear, **info}
+
+
 def cpu_baseline(nx, ny, iters):
     """The oracle (CPU FP64 NumPy/SciPy restatement) on the same workload: set-up excluded,
     one warm-up iteration (builds the first AMG hierarchy), then ``iters`` timed iterations."""
@@ -284,6 +313,7 @@ def main():
                     "log-uniform 16k-200k-cell batch)")
     ap.add_argument("--adjoint", type=int, default=0, help="also time the square-cylinder adjoint with this many "
                     "cells per body side (100: about 0.47M triangles)")
+    ap.add_argument("--sst", type=int, default=0, help="also time k-omega SST on the duct mesh (1) or not (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SIMPLEC path on a single MI355X "
@@ -320,6 +350,8 @@ def main():
         out["batch"] = batch_leg(args.batch, steps=10, warmup=3)
     if args.adjoint:
         out["adjoint"] = adjoint_leg(args.adjoint, tol=1e-9, max_iter=40000, gmres_tol=1e-8)
+    if args.sst:
+        out["sst"] = sst_leg(4 * args.nx, args.ny // 2, args.steps, args.warmup)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_iters)

@@ -432,6 +432,23 @@ def duct_mesh(nx: int = 1000, ny: int = 500, jitter: float = 0.2, seed: int = 0)
                              jitter=jitter, seed=seed)
 
 
+def geometric(n: int, length: float, first: float, start: float = 0.0) -> np.ndarray:
+    """Nodes of n cells on [start, start + length] whose sizes grow by a constant ratio from
+    ``first`` (``flowtrace/mesh/grading.py:9-19``): the ratio q solves first (q^n - 1) / (q - 1)
+    = length by Brent's method on (1 + 1e-12, 10); the partial sums are then scaled to the exact
+    length. Uniform when ``first`` is too large for growth."""
+    if n < 1:
+        raise ValueError("need at least one cell")
+    if first * n >= length:
+        return start + np.linspace(0.0, length, n + 1)
+    from scipy.optimize import brentq
+
+    q = brentq(lambda r: first * (r ** n - 1) / (r - 1) - length, 1.0 + 1e-12, 10.0)
+    x = np.concatenate([[0.0], np.cumsum(first * q ** np.arange(n))])
+    x *= length / x[-1]
+    return start + x
+
+
 def tanh_nodes(n: int, stretch: float = 2.0) -> np.ndarray:
     """n+1 nodes on [0,1] clustered at both ends: 0.5 (1 + tanh(s (2 eta - 1)) / tanh s)."""
     eta = np.linspace(0.0, 1.0, n + 1)

@@ -386,9 +386,10 @@ class SimpleSolver:
         tinfo = None
         if self.energy:
             tinfo = self._energy_step(L, s, M, B, R)
+        self._mark("correct+energy")
         if self.turb is not None:  # simple.py:662-669
             self.turb.solve(L, M, B, s, self.scal[_S_TURB:], self.scal[_S_TURB + 3:])
-        self._mark("correct+energy")
+            self._mark("turbulence")
         return infos, pinfo, tinfo
 
     def _finish(self, infos, pinfo, tinfo) -> float:

@@ -227,6 +227,26 @@ class DeviceSST:
         self.last_info = infos
         return infos
 
+    def wall_shear(self) -> np.ndarray:
+        """Wall shear stress magnitude on every boundary face, 0 off walls: the wall-function
+        viscosity times the wall-parallel velocity relative to the wall over y_b
+        (``turbulence.py:336-350``)."""
+        sv, fv = self.s, self.s.fv
+        L, s = nat.lib(), nat.stream_ptr()
+        M, B = ctypes.byref(fv.c_mesh), ctypes.byref(sv.c_bc)
+        nat.check(L.ftmi_sst_face_gamma(M, B, nat.ptr(self.mut), self.mu, 0, nat.ptr(self.gm_f),
+                                        nat.ptr(self.gm_b), s), "sst face viscosity")
+        nat.check(L.ftmi_sst_wall(M, B, nat.ptr(sv.U), nat.ptr(fv.yb), self.mu, self.nu, nat.ptr(self.iters),
+                                  nat.ptr(self.gm_b), nat.ptr(self.yplus), s), "sst wall function")
+        mu_w = self.gm_b.cpu().numpy()
+        U = sv.U.cpu().numpy().T
+        nb, ub = fv.nb_h, sv.ub_fixed_h
+        UP = U[fv.Pb_h]
+        ut = UP - np.einsum("fd,fd->f", UP, nb)[:, None] * nb
+        ut = ut - (ub - np.einsum("fd,fd->f", ub, nb)[:, None] * nb)
+        mag = np.linalg.norm(ut, axis=1)
+        return np.where(self.wall_b, mu_w * mag / np.maximum(fv.yb_h, 1e-300), 0.0)
+
     # ------------------------------------------------------------------ state
     def state(self) -> dict:
         return {"k": self.k.cpu().numpy(), "w": self.w.cpu().numpy(), "mut": self.mut.cpu().numpy()}

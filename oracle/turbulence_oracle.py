@@ -176,6 +176,16 @@ class SST:
         self.yplus = np.where(self.wall_b, yp, 0.0)
         return np.where(self.wall_b, mu_w, self.mu)
 
+    def wall_shear(self) -> np.ndarray:
+        """Wall shear stress magnitude on every boundary face, 0 off walls (``turbulence.py:336-350``)."""
+        s, fv = self.s, self.s.fv
+        UP = s.U[fv.Pb]
+        ut = UP - np.einsum("fd,fd->f", UP, fv.nb)[:, None] * fv.nb
+        ub = s.ub_fixed
+        ut = ut - (ub - np.einsum("fd,fd->f", ub, fv.nb)[:, None] * fv.nb)
+        mag = np.linalg.norm(ut, axis=1)
+        return np.where(self.wall_b, self.wall_shear_viscosity(s.U) * mag / np.maximum(self.yb, 1e-300), 0.0)
+
     def momentum_viscosity(self, U):
         """(interior faces: mu + interpolated mu_t, boundary faces: mu + mu_t of the owner, or
         mu_w on walls) (``turbulence.py:235-243``)."""

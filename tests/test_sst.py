@@ -9,7 +9,7 @@ import pytest
 
 import oracle.simplec_oracle as O
 from oracle.turbulence_oracle import B_SPALDING, KAPPA, spalding_uplus
-from sst_cases import GOLDEN, channel_hex, channel_quads, duct_tri, rel
+from sst_cases import DNS_YPLUS, GOLDEN, channel_hex, channel_profile, channel_quads, duct_tri, rel, sst_channel_dns
 
 
 def test_spalding_law_matches_the_reference_and_its_limits():
@@ -61,3 +61,18 @@ def test_oracle_follows_the_reference_trajectory(build, name, n_iter):
         np.testing.assert_allclose([raw[n] for n in names], g[f"it{it}_res"], rtol=1e-9)
     assert rel(t.wall_shear_viscosity(s.U), g["mu_w"]) < 1e-12
     assert rel(t.yplus, g["yplus"]) < 1e-12
+
+
+def test_oracle_reproduces_the_reference_channel_validation():
+    """The reference's SST pin (``tests/test_turbulence.py:53-58`` on
+    ``flowtrace/validation/turbulent.py:62-90``): the Re_tau = 392 half channel converges without
+    rollbacks, tau_w within 1 % of 1, and U+ within 5 % of the DNS from y+ = 1."""
+    mesh, case = sst_channel_dns(O)
+    s = O.Simplec(mesh, case)
+    while not s.finished:
+        s.step()
+    _, err = channel_profile(s.fv.xc, s.U[:, 0])
+    tau = s.turb.wall_shear()[s.turb.wall_b]
+    assert s.converged and not s.guard_events
+    assert abs(float(tau.mean()) - 1.0) < 0.01
+    assert np.abs(err[DNS_YPLUS >= 1.0]).max() < 5.0, err
