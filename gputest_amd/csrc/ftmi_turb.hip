@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_cell(int64_t n, const double* __
                 const double sij = 0.5 * (gU[(i * n + c) * DIM + j] + gU[(j * n + c) * DIM + i]);
                 s2 = s2 + sij * sij;
             }
-        const double Sc = sqrt(2.0 * s2);
+        const double smag = sqrt(2.0 * s2);  // |S| = sqrt(2 S_ij S_ij)
         double kw = gk[c * DIM] * gw[c * DIM];
 #pragma unroll
         for (int a = 1; a < DIM; ++a) kw += gk[c * DIM + a] * gw[c * DIM + a];
@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_cell(int64_t n, const double* __
         const double arg1 = dmin(a, C.four_rho_sw2 * kc / (cd * yc * yc));
         const double F1 = tanh(pow(arg1, 4.0));
         // sources (turbulence.py:419-433)
-        const double S2 = Sc * Sc;
+        const double S2 = smag * smag;
         const double Pk = dmin(mt * S2, C.ten_bstar_rho * kc * wc);
         gam_k[c] = C.mu + mix(F1, kSk1, kSk2) * mt;
         sck[c] = Pk * v;
@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_sst_cell(int64_t n, const double* __
         gam_w[c] = C.mu + mix(F1, kSw1, kSw2) * mt;
         scw[c] = Scw * v;
         spw[c] = Spw * v;
-        S[c] = Sc;
+        S[c] = smag;
     }
 }
 
@@ -282,12 +282,13 @@ __global__ void __launch_bounds__(kBlock) k_sst_face_gamma(int64_t nf, int64_t n
     }
 }
 
+__global__ void k_zero_int(int* x) { *x = 0; }
+
 // Spalding's law for every boundary face (turbulence.py:65-87, 399-417). The reference iterates
 // Newton on all faces together until every step is small; here pass 1 finds, per face, the first
 // iteration whose step meets the test and takes the maximum over faces, and pass 2 runs every
 // face exactly that many iterations, which is the same result wherever a face's test, once met,
 // stays met (Newton's steps shrink).
-__global__ void k_zero_int(int* x) { *x = 0; }
 
 struct Spalding {
     double eb;  // exp(-kappa B)
