@@ -163,6 +163,10 @@ class BatchJob:
     solver: str = "pressure-based"
 
 
+def _turbulent(kw: dict) -> bool:
+    return str(kw.get("turbulence", "laminar")).lower() not in ("laminar", "none")
+
+
 def _shared_key(kw: dict) -> str:
     """Case settings other than the boundary conditions: jobs that agree on them can share one
     solver (one viscosity, one set of relaxation factors and tolerances)."""
@@ -179,6 +183,7 @@ def plan_batch(jobs: Sequence[BatchJob], max_cells: int = 50_000_000) -> dict:
     if len(set(names)) != len(names):
         raise ValueError("job names must be unique")
     rows, meshes, errors = [], {}, {}
+    by_name_case = {j.name: j.case for j in jobs}
     for j in jobs:
         try:
             if j.solver != "pressure-based":
@@ -190,7 +195,9 @@ def plan_batch(jobs: Sequence[BatchJob], max_cells: int = 50_000_000) -> dict:
     groups: List[List[str]] = []
     by_key: Dict[str, List[dict]] = {}
     for r in rows:
-        by_key.setdefault(r["key"], []).append(r)
+        # a turbulent case runs alone: its wall distance must see its own walls only
+        key = r["name"] if _turbulent(by_name_case[r["name"]]) else r["key"]
+        by_key.setdefault(key, []).append(r)
     for key, rs in by_key.items():
         cur, size = [], 0
         for r in sorted(rs, key=lambda r: -r["cells"]):
@@ -240,7 +247,7 @@ def run_batch(jobs: Sequence[BatchJob], precision: str = "fp64", device="cuda", 
             offs = np.concatenate([[0], np.cumsum([meshes[n].n_cells for n in group])])
             kw.setdefault("precision", precision)
             solver = SimpleSolver(cat, UnstructuredCase(boundaries=bnd, **kw), device)
-            solver.use_graph = use_graph
+            solver.use_graph = use_graph and solver.turb is None  # the SST iteration is not graph-recorded
             if closed:
                 solver.set_reference_cells(offs[closed])
             t1 = time.perf_counter()

@@ -662,14 +662,16 @@ class SimpleSolver:
         return m_pres * self.pb_fixed_h + (1.0 - m_pres) * pP
 
     def wall_forces(self) -> dict:
-        """Pressure + viscous force on every wall zone (``simple.py:970-1001``)."""
+        """Pressure + viscous force on every wall zone (``simple.py:970-1001``); the viscous part
+        uses the wall viscosity of the momentum equation (laminar, or the SST wall function)."""
         fv = self.fv
         pb = self._pressure_bvals_h()
         U = self.U.cpu().numpy().T
         du = U[fv.Pb_h] - self.ub_fixed_h
         du_t = du - np.einsum("fd,fd->f", du, fv.nb_h)[:, None] * fv.nb_h
         f_p = pb[:, None] * fv.Sb_h
-        f_v = (self.case.viscosity * fv.alpha_b_h)[:, None] * du_t
+        mu_b = self.turb.boundary_viscosity() if self.turb is not None else np.full(fv.n_bf, self.case.viscosity)
+        f_v = (mu_b * fv.alpha_b_h)[:, None] * du_t
         out = {}
         for z in self.mesh.boundary_zones():
             m = (fv.bzone == z.id) & (self.bkind_h == WALL)

@@ -227,10 +227,9 @@ class DeviceSST:
         self.last_info = infos
         return infos
 
-    def wall_shear(self) -> np.ndarray:
-        """Wall shear stress magnitude on every boundary face, 0 off walls: the wall-function
-        viscosity times the wall-parallel velocity relative to the wall over y_b
-        (``turbulence.py:336-350``)."""
+    def boundary_viscosity(self) -> np.ndarray:
+        """The viscosity the momentum equation uses on every boundary face for the current
+        velocity (host array): mu + mu_t, or the wall-function value on walls."""
         sv, fv = self.s, self.s.fv
         L, s = nat.lib(), nat.stream_ptr()
         M, B = ctypes.byref(fv.c_mesh), ctypes.byref(sv.c_bc)
@@ -238,7 +237,14 @@ class DeviceSST:
                                         nat.ptr(self.gm_b), s), "sst face viscosity")
         nat.check(L.ftmi_sst_wall(M, B, nat.ptr(sv.U), nat.ptr(fv.yb), self.mu, self.nu, nat.ptr(self.iters),
                                   nat.ptr(self.gm_b), nat.ptr(self.yplus), s), "sst wall function")
-        mu_w = self.gm_b.cpu().numpy()
+        return self.gm_b.cpu().numpy()
+
+    def wall_shear(self) -> np.ndarray:
+        """Wall shear stress magnitude on every boundary face, 0 off walls: the wall-function
+        viscosity times the wall-parallel velocity relative to the wall over y_b
+        (``turbulence.py:336-350``)."""
+        sv, fv = self.s, self.s.fv
+        mu_w = self.boundary_viscosity()
         U = sv.U.cpu().numpy().T
         nb, ub = fv.nb_h, sv.ub_fixed_h
         UP = U[fv.Pb_h]

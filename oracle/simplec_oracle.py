@@ -832,7 +832,9 @@ class Simplec:
         du = self.U[fv.Pb] - self.ub_fixed
         du_t = du - np.einsum("fd,fd->f", du, fv.nb)[:, None] * fv.nb
         f_p = pb[:, None] * fv.Sb
-        f_v = (self.case.viscosity * fv.alpha_b)[:, None] * du_t
+        mu_b = self.turb.momentum_viscosity(self.U)[1] if self.turb is not None \
+            else np.full(fv.n_bf, self.case.viscosity)
+        f_v = (mu_b * fv.alpha_b)[:, None] * du_t
         out = {}
         for z in self.mesh.boundary_zones():
             m = (fv.bzone == z.id) & (self.bkind == WALL)

@@ -123,6 +123,11 @@ def test_one_sst_iteration_from_the_same_state_matches_the_oracle(build, name, n
     print(name, "one iteration from the same state", {f: f"{v:.1e}" for f, v in e.items()}, s.last_linear,
           o.last_linear)
     assert max(e.values()) < 1e-10, e
+    # wall forces with the wall-function viscosity (simple.py:970-1001)
+    fd, fo = s.wall_forces(), o.wall_forces()
+    assert sorted(fd) == sorted(fo)
+    for z in fo:
+        assert rel(fd[z]["total"], fo[z]["total"]) < 1e-10, (z, fd[z], fo[z])
     ro, rd = o.residuals()["raw"], s.residuals()["raw"]
     for n in ("k", "omega"):
         assert abs(rd[n] - ro[n]) <= 1e-9 * abs(ro[n]), (n, rd[n], ro[n])

@@ -85,3 +85,14 @@ def test_plan_groups_compatible_jobs_largest_first(tmp_path):
     assert small["groups"] == [["b"], ["d", "a"]]
     with pytest.raises(ValueError):
         plan_batch([BatchJob("x", a), BatchJob("x", b)])
+
+
+def test_turbulent_jobs_run_alone(tmp_path):
+    """A k-omega SST job gets a launch of its own (its wall distance must see only its own walls),
+    even beside identical settings."""
+    a, b = _msh(tmp_path, "a", 8), _msh(tmp_path, "b", 12)
+    sst = {"viscosity": 0.01, "turbulence": "k-omega-sst"}
+    jobs = [BatchJob("a", a, dict(sst)), BatchJob("b", b, dict(sst)), BatchJob("c", a, {"viscosity": 0.01}),
+            BatchJob("d", b, {"viscosity": 0.01})]
+    plan = plan_batch(jobs)
+    assert sorted(plan["groups"]) == [["a"], ["b"], ["d", "c"]]
