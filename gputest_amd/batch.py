@@ -247,7 +247,7 @@ def run_batch(jobs: Sequence[BatchJob], precision: str = "fp64", device="cuda", 
             offs = np.concatenate([[0], np.cumsum([meshes[n].n_cells for n in group])])
             kw.setdefault("precision", precision)
             solver = SimpleSolver(cat, UnstructuredCase(boundaries=bnd, **kw), device)
-            solver.use_graph = use_graph and solver.turb is None  # the SST iteration is not graph-recorded
+            solver.use_graph = use_graph
             if closed:
                 solver.set_reference_cells(offs[closed])
             t1 = time.perf_counter()

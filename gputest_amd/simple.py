@@ -241,8 +241,6 @@ class SimpleSolver:
             return 0.0
         self._events = []
         if self.use_graph:
-            if self.turb is not None:
-                raise NotImplementedError("graph replay of the SST outer iteration is not built; set use_graph=False")
             return self._graph_step()
         self._mark("start")
         out = self._launch(rebuild=True)

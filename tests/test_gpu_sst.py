@@ -11,6 +11,7 @@ reference's trajectories (``tests/golden/sst_*.npz``) and the oracle.
 * The wall-function viscosity and y+ of the final velocity as the reference's.
 * The reference's own validation: the Re_tau = 392 half channel converges without rollbacks
   to tau_w within 1 % of 1 and U+ within 5 % of the DNS, as the oracle's converged solution.
+* Outer iterations replayed from HIP graphs equal eager ones bit for bit.
 """
 import ctypes
 
@@ -157,3 +158,20 @@ def test_sst_channel_matches_dns_and_the_oracle():
     assert abs(float(tau.mean()) - 1.0) < 0.01
     assert np.abs(err[DNS_YPLUS >= 1.0]).max() < 5.0, err
     assert rel(U, Uo) < 5e-4
+
+
+def test_sst_graph_replays_equal_eager_iterations():
+    """25 iterations of the heated SST duct, spanning AMG rebuilds and re-recordings."""
+    mesh, case = duct_tri(C, 25)
+    eager = SimpleSolver(mesh, case, DEV)
+    graph = SimpleSolver(mesh, duct_tri(C, 25)[1], DEV)
+    graph.use_graph = True
+    for _ in range(25):
+        eager.step()
+        graph.step()
+    assert graph.graph_captures >= 2
+    for name in ("U", "p", "F", "Fb", "T"):
+        assert torch.equal(getattr(eager, name), getattr(graph, name)), name
+    for name in ("k", "w", "mut"):
+        assert torch.equal(getattr(eager.turb, name), getattr(graph.turb, name)), name
+    assert eager.residuals()["raw"] == graph.residuals()["raw"]
