@@ -16,10 +16,14 @@ operator (SA-AMG-PCG). The adjoint system is solved by restarted GMRES on vector
 products of Phi evaluated once at w* (the forward iterations are not stored: the single
 step is recomputed in the backward pass). ``drag_gradient`` returns dJ/d(inlet profile
 control values) and dJ/d(nu); ``AdjointDrag`` wraps forward solve and adjoint backward as a
-``torch.autograd.Function``.
+``torch.autograd.Function``. With the energy equation the state also holds T and its lagged
+gradient, Phi also takes the energy step (``simple.py:778-813``, after the corrections, with
+the corrected fluxes), and ``heat_gradient`` returns d(heat flow through a boundary zone,
+``simple.py:1003-1031``)/d(inlet profile control values, the temperature of a fixed-temperature
+wall zone).
 
-Scope: laminar, no energy, no periodic pairs or body force, velocity inlets with a profile
-given by ``InletProfile``.
+Scope: laminar, no periodic pairs or body force, velocity inlets with a profile given by
+``InletProfile``.
 """
 from __future__ import annotations
 
@@ -36,6 +40,7 @@ from .amg import SmoothedAggregationAMG
 from .krylov import Workspace, bicgstab_ldu, pcg_ldu
 
 WALL, INLET, OUTLET, PRESSURE, SYMMETRY = range(5)
+T_ADIABATIC, T_FIXED, T_FLUX = range(3)
 
 
 class InletProfile:
@@ -114,10 +119,17 @@ class ResidualMap:
     """One SIMPLEC outer iteration Phi(w; beta) in torch on the state of ``solver``."""
 
     def __init__(self, solver, zone: str = "cylinder", inlet: str = "inlet", profile: Optional[InletProfile] = None,
-                 solve_tol: float = 1e-13):
+                 solve_tol: float = 1e-13, objective: str = "drag", wall_T_zone: Optional[str] = None):
         c = solver.case
-        if solver.energy or getattr(c, "periodic", None) or any(c.body_force) or any(c.periodic_pressure_gradient):
-            raise NotImplementedError("the adjoint covers laminar flow without energy, periodic pairs or forcing")
+        if getattr(c, "periodic", None) or any(c.body_force) or any(c.periodic_pressure_gradient):
+            raise NotImplementedError("the adjoint covers flow without periodic pairs or forcing")
+        if getattr(solver, "turb", None) is not None:
+            raise NotImplementedError("the adjoint covers laminar flow")
+        if objective not in ("drag", "heat"):
+            raise ValueError("objective must be 'drag' or 'heat'")
+        if objective == "heat" and not solver.energy:
+            raise ValueError("the heat-flow objective needs the energy equation")
+        self.objective_kind = objective
         self.s, self.case, self.fv = solver, c, solver.fv
         fv, dev = solver.fv, solver.device
         self.device, self.dim, self.nc = dev, solver.dim, solver.nc
@@ -140,7 +152,26 @@ class ResidualMap:
         self.singular = solver.singular_p
         zb = fv.bzone
         zid = solver.mesh.zone_by_name(zone).id
-        self.zone_mask = torch.as_tensor((zb == zid) & (solver.bkind_h == WALL), device=dev).to(torch.float64)
+        on_zone = (zb == zid) if objective == "heat" else (zb == zid) & (solver.bkind_h == WALL)
+        self.zone_mask = torch.as_tensor(on_zone, device=dev).to(torch.float64)
+        self.energy = bool(solver.energy)
+        if self.energy:
+            tk = np.asarray(solver.tkind_h)
+            self.m_tfix = torch.as_tensor(tk == T_FIXED, device=dev).to(torch.float64)
+            self.m_tflux = torch.as_tensor(tk == T_FLUX, device=dev).to(torch.float64)
+            self.m_out = torch.as_tensor(solver.bkind_h == OUTLET, device=dev)
+            self.tval0 = torch.as_tensor(np.asarray(solver.tval_h, np.float64), device=dev)
+            self.k_over_cp, self.cp, self.k = float(c.conductivity / c.cp), float(c.cp), float(c.conductivity)
+            self.relax_T = float(c.relax_T)
+            self.Sb_mag = fv.Sb_mag
+            self.wall_T_faces = None
+            if wall_T_zone is not None:
+                wz = solver.mesh.zone_by_name(wall_T_zone).id
+                sel = (zb == wz) & (tk == T_FIXED)
+                if not sel.any():
+                    raise ValueError(f"zone {wall_T_zone!r} has no fixed-temperature faces")
+                self.wall_T_faces = torch.as_tensor(sel, device=dev).to(torch.float64)
+                self.wall_T0 = float(np.asarray(solver.tval_h)[sel][0])
         iz = solver.mesh.zone_by_name(inlet)
         self.inlet_faces = np.flatnonzero(zb == iz.id)
         self.profile = profile
@@ -166,18 +197,28 @@ class ResidualMap:
     # ---------------------------------------------------------------- state and parameters
     def state(self) -> List[torch.Tensor]:
         """The converged state as leaf tensors: U (n, dim), p, F, Fb, grad p (n, dim),
-        grad U (dim, n, dim)."""
+        grad U (dim, n, dim), and with energy T and grad T (n, dim)."""
         s = self.s
         U = s.U.t().contiguous().clone()
-        return [t.detach().clone().requires_grad_(True)
-                for t in (U, s.p, s.F, s.Fb, s.gp, s.gU)]
+        fields = (U, s.p, s.F, s.Fb, s.gp, s.gU) + ((s.T, s.gT) if self.energy else ())
+        return [t.detach().clone().requires_grad_(True) for t in fields]
 
-    def params(self, u_ctrl=None, nu=None) -> List[torch.Tensor]:
+    def params(self, u_ctrl=None, nu=None, wall_T=None) -> List[torch.Tensor]:
+        """[inlet control values, nu] and, with a fixed-temperature wall zone, its temperature."""
         u = torch.as_tensor(self.profile.values if u_ctrl is None else u_ctrl, device=self.device,
                             dtype=torch.float64)
         v = torch.as_tensor(self.case.viscosity / self.case.density if nu is None else nu, device=self.device,
                             dtype=torch.float64)
-        return [u.detach().clone().requires_grad_(True), v.detach().clone().reshape(()).requires_grad_(True)]
+        out = [u.detach().clone().requires_grad_(True), v.detach().clone().reshape(()).requires_grad_(True)]
+        if self.energy and self.wall_T_faces is not None:
+            tw = torch.as_tensor(self.wall_T0 if wall_T is None else wall_T, device=self.device, dtype=torch.float64)
+            out.append(tw.detach().clone().reshape(()).requires_grad_(True))
+        return out
+
+    def _tval(self, beta):
+        if self.wall_T_faces is None or len(beta) < 3:
+            return self.tval0
+        return self.tval0 * (1.0 - self.wall_T_faces) + beta[2] * self.wall_T_faces
 
     def _ub_fixed(self, beta):
         if self.profile is None:
@@ -252,7 +293,7 @@ class ResidualMap:
 
     # ---------------------------------------------------------------- Phi and J
     def step(self, w: List[torch.Tensor], beta: List[torch.Tensor]) -> List[torch.Tensor]:
-        U, p, F, Fb, gp_lag, gU_lag = w
+        U, p, F, Fb, gp_lag, gU_lag = w[:6]
         gp_lag = self.gp_free * gp_lag + (1.0 - self.gp_free) * gp_lag.detach()
         mu = self.rho * beta[1]
         ub_fixed = self._ub_fixed(beta)
@@ -318,18 +359,72 @@ class ResidualMap:
             p_new = p_new - p_new.mean()
         gpp = self._ls(pp, (1.0 - self.m_pres) * pp[Pb])
         U_new = Ustar - dP[:, None] * gpp
-        return [U_new, p_new, F_new, Fb_new, gp, torch.stack(gU, dim=0)]
+        out = [U_new, p_new, F_new, Fb_new, gp, torch.stack(gU, dim=0)]
+        if self.energy:
+            out += self._energy(w[6], w[7], F_new, Fb_new, beta)
+        return out
+
+    def _energy(self, T, gT_lag, F, Fb, beta):
+        """The energy step (``simple.py:778-813``) with the corrected fluxes: boundary
+        temperatures (fixed faces, backflow at outlets, else the owner value reconstructed with
+        the lagged gradient), the LS gradient, upwind convection with the deferred second-order
+        correction, k/cp diffusion with the non-orthogonal term, the heat-flux source, relaxation,
+        and the exact solve."""
+        P, N, Pb = self.P, self.N, self.Pb
+        relu = torch.nn.functional.relu
+        tval = self._tval(beta)
+        TP = T[Pb] + torch.sum(gT_lag[Pb] * self.dbt, dim=1)
+        Tb = self.m_tfix * tval + (1.0 - self.m_tfix) * TP
+        back = (self.m_out & (Fb < 0)).to(torch.float64)
+        Tb = back * tval + (1.0 - back) * Tb
+        gT = self._ls(T, Tb)
+        gam = self.k_over_cp
+        D = gam * self.alpha
+        aPN, aNP = -(D + relu(-F)), -(D + relu(F))
+        Db = gam * self.alpha_b * self.m_tfix
+        diag = self._sc(D + relu(F), D + relu(-F), Db + relu(Fb))
+        rhs = self._sc(bnd=(Db + relu(-Fb)) * Tb)
+        tn = gam * torch.sum(self._interp(gT) * self.Tv, dim=1)
+        rhs = rhs + self._sc(tn, -tn)
+        if self.second_order:
+            phP, phN = T[P], T[N]
+            hoP = phP + torch.sum(gT[P] * self.dPf, dim=1)
+            hoN = phN + torch.sum(gT[N] * self.dNf, dim=1)
+            up = F >= 0
+            ho = torch.minimum(torch.maximum(torch.where(up, hoP, hoN), torch.minimum(phP, phN)),
+                               torch.maximum(phP, phN))
+            dc = F * (ho - torch.where(up, phP, phN))
+            rhs = rhs - self._sc(dc, -dc)
+        rhs = rhs + self._sc(bnd=self.m_tflux * tval * self.Sb_mag / self.cp)
+        aT = self.relax_T
+        b = rhs + ((1.0 - aT) / aT) * diag * T
+        T_new = _MomentumSolve.apply(diag / aT, aPN, aNP, b, T.detach(), self)
+        return [T_new, gT]
 
     def objective(self, w: List[torch.Tensor], beta: List[torch.Tensor]) -> torch.Tensor:
         """Drag: x-component of the pressure and viscous force on the wall zone
-        (``simple.py:970-1001``)."""
-        U, p, F, Fb, gp, gU = w
+        (``simple.py:970-1001``); or the heat flow into the domain through the zone, conduction
+        plus enthalpy (``simple.py:1003-1031``)."""
+        if self.objective_kind == "heat":
+            return self._heat_flow(w, beta)
+        U, p, F, Fb, gp, gU = w[:6]
         mu = self.rho * beta[1]
         pb = self._pbvals(p, gp)
         du = U[self.Pb] - self._ub_fixed(beta)
         du_t = du - torch.sum(du * self.nb, dim=1)[:, None] * self.nb
         fx = pb * self.Sb[:, 0] + mu * self.alpha_b * du_t[:, 0]
         return torch.sum(self.zone_mask * fx)
+
+    def _heat_flow(self, w, beta):
+        Fb, T = w[3], w[6]
+        tval = self._tval(beta)
+        TP = T[self.Pb]
+        Tb = self.m_tfix * tval + (1.0 - self.m_tfix) * TP
+        back = (self.m_out & (Fb < 0)).to(torch.float64)
+        Tb = back * tval + (1.0 - back) * Tb
+        cond = self.k * self.alpha_b * (Tb - TP) * self.m_tfix + self.m_tflux * tval * self.Sb_mag
+        q = cond - self.cp * Fb * Tb
+        return torch.sum(self.zone_mask * q)
 
     # ---------------------------------------------------------------- transposed solves
     def _off(self, a_own, a_nei):
@@ -447,12 +542,26 @@ def drag_gradient(solver, profile: InletProfile, zone: str = "cylinder", inlet: 
     ``solver`` (run with ``profile`` as its inlet and nu = viscosity / density).
     Returns (grad_u (n_ctrl,), grad_nu, J, AdjointInfo)."""
     with torch.enable_grad():  # also when called from an autograd backward pass
-        return _drag_gradient(solver, profile, zone, inlet, tol, restart, maxiter, rmap)
+        rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile)
+        grads, J, info = _gradient(rm, tol, restart, maxiter)
+        return grads[0], grads[1], J, info
 
 
-def _drag_gradient(solver, profile, zone, inlet, tol, restart, maxiter, rmap):
+def heat_gradient(solver, profile: InletProfile, zone: str = "outlet", wall_T_zone: str = "cylinder",
+                  inlet: str = "inlet", tol: float = 1e-10, restart: int = 200, maxiter: int = 1000,
+                  rmap: Optional[ResidualMap] = None):
+    """dJ/d(u_ctrl) and dJ/d(T_wall) of the heat flow J into the domain through ``zone`` at the
+    converged state of ``solver`` (energy on; ``wall_T_zone`` a fixed-temperature wall).
+    Returns (grad_u (n_ctrl,), grad_T_wall, J, AdjointInfo)."""
+    with torch.enable_grad():
+        rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile, objective="heat",
+                                                       wall_T_zone=wall_T_zone)
+        grads, J, info = _gradient(rm, tol, restart, maxiter)
+        return grads[0], grads[2], J, info
+
+
+def _gradient(rm, tol, restart, maxiter):
     t0 = time.perf_counter()
-    rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile)
     w = rm.state()
     beta = rm.params()
     # Phi and J once at the fixed point: their graphs are reused by every product
@@ -475,7 +584,7 @@ def _drag_gradient(solver, profile, zone, inlet, tol, restart, maxiter, rmap):
     if rm.device.type == "cuda":
         torch.cuda.synchronize(rm.device)
     info = AdjointInfo(its, rel, time.perf_counter() - t0, rm.vjp_count)
-    return grads[0].detach(), grads[1].detach(), float(J.detach()), info
+    return [g.detach() if torch.is_tensor(g) else torch.as_tensor(g) for g in grads], float(J.detach()), info
 
 
 class AdjointDrag(torch.autograd.Function):
@@ -500,4 +609,4 @@ class AdjointDrag(torch.autograd.Function):
         return g * gu.to(g.device), g * gnu.to(g.device), None, None
 
 
-__all__ = ["InletProfile", "ResidualMap", "drag_gradient", "AdjointDrag", "gmres", "AdjointInfo"]
+__all__ = ["InletProfile", "ResidualMap", "drag_gradient", "heat_gradient", "AdjointDrag", "gmres", "AdjointInfo"]

@@ -5,7 +5,8 @@ in the attributes ``gputest_amd.adjoint.ResidualMap`` reads, with the two linear
 the adjoint products done by SciPy's sparse direct solver on the CPU instead of the HIP
 Krylov solvers. Used by ``tests/`` to check the device adjoint against (1) the fixed-point
 property of the restated outer iteration at the oracle's converged state, (2) central
-finite differences of the oracle's converged drag and (3) this CPU gradient.
+finite differences of the oracle's converged drag (and, with the energy equation, outlet heat
+flow) and (3) this CPU gradient.
 """
 from __future__ import annotations
 
@@ -31,6 +32,9 @@ def solver_view(o, device="cpu"):
     v.singular_p, v.relax_p, v.relax_u, v.algo = o.singular_p, o.relax_p, o.relax_u, o.algo
     v.U, v.p, v.F, v.Fb = t(o.U.T), t(o.p), t(o.F), t(o.Fb)
     v.gp, v.gU = t(o._gp), t(np.stack(o._gU, axis=0))
+    if o.energy:
+        v.tkind_h, v.tval_h = o.tkind, o.tval
+        v.T, v.gT = t(o.T), t(o._gT)
     return v
 
 

@@ -22,3 +22,16 @@ def case(mod, u, nu, tol=1e-14, max_iter=6000):
     c = mod.Case(boundaries={"inlet": B("inlet", profile=prof), "outlet": B("outlet"), "bottom": B("symmetry"),
                              "top": B("symmetry"), "cylinder": B("wall")}, viscosity=nu, max_iter=max_iter, tol=tol)
     return c, prof
+
+
+def energy_case(mod, u, nu, T_wall, tol=1e-14, max_iter=8000):
+    """The same flow with energy: inlet and outlet at T = 300, the cylinder held at T_wall,
+    adiabatic symmetry sides, Pr = 0.7 (k = nu rho c_p / 0.7 with rho = c_p = 1)."""
+    prof = InletProfile(-HALF, HALF, u)
+    B = mod.Boundary if hasattr(mod, "Boundary") else mod.BoundarySpec
+    bnd = {"inlet": B("inlet", profile=prof, value=300.0), "outlet": B("outlet", value=300.0),
+           "bottom": B("symmetry"), "top": B("symmetry"), "cylinder": B("wall", thermal="temperature", value=T_wall)}
+    kw = dict(viscosity=nu, energy=True, conductivity=nu / 0.7, cp=1.0, init_T=300.0, max_iter=max_iter, tol=tol)
+    if hasattr(mod, "Boundary"):
+        return mod.Case(boundaries=bnd, **kw), prof
+    return mod.UnstructuredCase(boundaries=bnd, **kw), prof

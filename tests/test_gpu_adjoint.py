@@ -1,6 +1,7 @@
 """The discrete adjoint on the GPU (HIP forward solver, transposed momentum BiCGSTAB and
 pressure AMG-PCG in the adjoint products) against the CPU reference adjoint on the oracle's
-forward, and against central finite differences of the GPU forward; the autograd wrapper."""
+forward, and against central finite differences of the GPU forward, for the drag and (with the
+energy equation) the outlet heat flow; the autograd wrapper."""
 import numpy as np
 import pytest
 import torch
@@ -8,9 +9,9 @@ import torch
 import oracle.simplec_oracle as O
 from oracle.adjoint_oracle import CpuResidualMap, solver_view
 from gputest_amd import case as C
-from gputest_amd.adjoint import AdjointDrag, ResidualMap, drag_gradient
+from gputest_amd.adjoint import AdjointDrag, ResidualMap, drag_gradient, heat_gradient
 from gputest_amd.simple import SimpleSolver
-from tests.adjoint_cases import case, small_mesh, u_ctrl0
+from tests.adjoint_cases import case, energy_case, small_mesh, u_ctrl0
 
 pytestmark = pytest.mark.gpu
 NU = 1.0 / 40.0
@@ -82,3 +83,53 @@ def test_autograd_function_returns_the_adjoint_gradient():
     assert abs(float(J) - J2) <= 1e-12 * abs(J2)
     assert torch.allclose(u.grad, 2.0 * gu, rtol=1e-8, atol=0)
     assert abs(float(nu.grad) - 2.0 * float(gnu)) <= 1e-8 * abs(float(gnu))
+
+
+T_WALL = 350.0
+
+
+def _gpu_energy(mesh, u, T_wall):
+    c, prof = energy_case(C, u, NU, T_wall)
+    s = SimpleSolver(mesh, c, DEV)
+    while not s.finished:
+        s.step()
+    assert s.converged
+    return s, prof
+
+
+def test_gpu_heat_adjoint_matches_cpu_adjoint_and_central_differences():
+    """d(outlet heat flow)/d(inlet control values, cylinder temperature)."""
+    mesh, u0 = small_mesh(), u_ctrl0()
+    s, prof = _gpu_energy(mesh, u0, T_WALL)
+    gu, gT, J, info = heat_gradient(s, prof)
+    print("gpu heat adjoint", info, "J", J, "dJ/dT_wall", float(gT))
+    assert info.residual < 1e-9
+    assert abs(J - s.heat_flow()["outlet"]) <= 1e-12 * abs(J)
+    c, prof_o = energy_case(O, u0, NU, T_WALL)
+    o = O.Simplec(mesh, c)
+    while not o.finished:
+        o.step()
+    rm = CpuResidualMap(solver_view(o), "outlet", "inlet", prof_o, objective="heat", wall_T_zone="cylinder")
+    gu_c, gT_c, J_c, info_c = heat_gradient(None, prof_o, rmap=rm, tol=1e-11)
+    print("cpu heat adjoint", info_c, "J", J_c)
+    assert abs(J - J_c) <= 1e-10 * abs(J_c)
+    ref = np.r_[gu_c.numpy(), float(gT_c)]
+    got = np.r_[gu.cpu().numpy(), float(gT)]
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("gpu vs cpu heat adjoint", err)
+    assert err < 1e-7
+    h = 1e-6
+
+    def heat(u, T_wall):
+        return _gpu_energy(mesh, u, T_wall)[0].heat_flow()["outlet"]
+
+    k = 31
+    up, um = u0.copy(), u0.copy()
+    up[k] += h
+    um[k] -= h
+    fd = (heat(up, T_WALL) - heat(um, T_WALL)) / (2 * h)
+    print("u", k, float(gu[k]), fd)
+    assert abs(float(gu[k]) - fd) <= 1e-5 * abs(fd)
+    fd = (heat(u0, T_WALL * (1 + h)) - heat(u0, T_WALL * (1 - h))) / (2 * h * T_WALL)
+    print("T_wall", float(gT), fd)
+    assert abs(float(gT) - fd) <= 1e-5 * abs(fd)
