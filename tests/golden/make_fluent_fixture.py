@@ -1,19 +1,33 @@
-"""Copy the reference's regression mesh into the test fixtures, gzipped (data only).
+"""Copy the reference's Fluent mesh files into the test fixtures, gzipped (data only).
 
-The reference's tests hold ``tests/data/SquareCylinder-Coarse.msh`` (an ASCII Fluent mesh
-written by GAMBIT: 5 824 quads, square cylinder D = 4 in a channel; see that folder's
-README). The fixture is the same bytes, gzip-compressed; the reader takes .gz directly.
+* ``tests/data/SquareCylinder-Coarse.msh``: the reference's regression mesh, an ASCII Fluent
+  mesh written by GAMBIT (5 824 quads, square cylinder D = 4 in a channel; see that folder's
+  README).
+* ``examples/heated_duct_tri.msh`` (ASCII, 2 400 skewed triangles, zones inlet, outlet, floor,
+  ceiling), ``examples/cavity_quad.msh`` (binary, double precision, a 40 x 40 cavity with zone
+  ``lid``) and ``examples/elbow-mix.msh.txt`` (ASCII from GAMBIT, a 2-D mixing elbow with two
+  inlets): the reference's example meshes (``examples/README.md``).
+
+Each fixture is the same bytes, gzip-compressed; the reader takes .gz directly.
 Run from the repository root: ``python tests/golden/make_fluent_fixture.py``.
 """
 import gzip
 import os
 
-SRC = "/root/reference/tests/data/SquareCylinder-Coarse.msh"
-DST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "SquareCylinder-Coarse.msh.gz")
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+FILES = {
+    "tests/data/SquareCylinder-Coarse.msh": "SquareCylinder-Coarse.msh.gz",
+    "examples/heated_duct_tri.msh": "example_heated_duct_tri.msh.gz",
+    "examples/cavity_quad.msh": "example_cavity_quad.msh.gz",
+    "examples/elbow-mix.msh.txt": "example_elbow_mix.msh.gz",
+}
 
 if __name__ == "__main__":
-    with open(SRC, "rb") as fh:
-        data = fh.read()
-    with open(DST, "wb") as fh:
-        fh.write(gzip.compress(data, 9, mtime=0))
-    print(f"wrote {DST}: {len(data)} bytes -> {os.path.getsize(DST)} gzipped")
+    for src, dst in FILES.items():
+        with open(os.path.join(REF, src), "rb") as fh:
+            data = fh.read()
+        out = os.path.join(HERE, dst)
+        with open(out, "wb") as fh:
+            fh.write(gzip.compress(data, 9, mtime=0))
+        print(f"wrote {out}: {len(data)} bytes -> {os.path.getsize(out)} gzipped")

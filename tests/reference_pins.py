@@ -1,0 +1,152 @@
+"""The reference's physics pins of the SIMPLE/SIMPLEC path (``tests/test_unstructured.py:27-71``)
+and the cases of its example meshes (``examples/README.md``), written once for the oracle
+(``mod`` = ``oracle.simplec_oracle``) and for the device solver (``mod`` = ``gputest_amd.case``).
+
+The lid-driven cavity is checked against the centreline velocities of Ghia, Ghia & Shin (1982),
+J. Comput. Phys. 48:387-411, Tables I and II at Re = 100: the published data, as the reference
+tabulates them (``flowtrace/validation/reference.py:12-30``). The comparison interpolates the
+simulated profile, padded with the wall values, at Ghia's points and takes the largest deviation
+(``reference.py:49-66``)."""
+import numpy as np
+from scipy.interpolate import griddata
+
+from gputest_amd.mesh import cartesian_mesh_2d, extrude_to_3d
+
+GHIA_Y = np.array([1.0000, 0.9766, 0.9688, 0.9609, 0.9531, 0.8516, 0.7344, 0.6172, 0.5000, 0.4531, 0.2813,
+                   0.1719, 0.1016, 0.0703, 0.0625, 0.0547, 0.0000])
+GHIA_U100 = np.array([1.00000, 0.84123, 0.78871, 0.73722, 0.68717, 0.23151, 0.00332, -0.13641, -0.20581,
+                      -0.21090, -0.15662, -0.10150, -0.06434, -0.04775, -0.04192, -0.03717, 0.00000])
+GHIA_X = np.array([1.0000, 0.9688, 0.9609, 0.9531, 0.9453, 0.9063, 0.8594, 0.8047, 0.5000, 0.2344, 0.2266,
+                   0.1563, 0.0938, 0.0781, 0.0703, 0.0625, 0.0000])
+GHIA_V100 = np.array([0.00000, -0.05906, -0.07391, -0.08864, -0.10313, -0.16914, -0.22445, -0.24533, 0.05454,
+                      0.17527, 0.17507, 0.16077, 0.12317, 0.10890, 0.10091, 0.09233, 0.00000])
+
+
+def is_oracle(mod):
+    return hasattr(mod, "Simplec")
+
+
+def make_case(mod, boundaries, **kw):
+    if is_oracle(mod):
+        return mod.Case(boundaries={k: mod.Boundary(**v) for k, v in boundaries.items()}, **kw)
+    return mod.UnstructuredCase(boundaries={k: mod.BoundarySpec(**v) for k, v in boundaries.items()}, **kw)
+
+
+def solver(mod, mesh, case):
+    if is_oracle(mod):
+        return mod.Simplec(mesh, case)
+    from gputest_amd.simple import SimpleSolver
+
+    return SimpleSolver(mesh, case, "cuda")
+
+
+def run(s):
+    while not s.finished:
+        s.step()
+    return s
+
+
+def centres(s):
+    return s.fv.xc if is_oracle_solver(s) else s.cell_centres()
+
+
+def velocity(s):
+    """Cell velocities as (cells, dim) on the host."""
+    U = s.U
+    return U if isinstance(U, np.ndarray) else U.detach().cpu().numpy().T
+
+
+def temperature(s):
+    T = s.T
+    return T if isinstance(T, np.ndarray) else T.detach().cpu().numpy()
+
+
+def mass_balance(s):
+    """Net boundary flux per boundary zone (``simple.py`` ``mass_balance``)."""
+    if not is_oracle_solver(s):
+        return s.mass_balance()
+    fv = s.fv
+    return {z.name: float(s.Fb[fv.bzone == z.id].sum()) for z in s.mesh.boundary_zones() if (fv.bzone == z.id).any()}
+
+
+def is_oracle_solver(s):
+    return isinstance(s.U, np.ndarray)
+
+
+def ghia_errors(s):
+    """``tests/test_unstructured.py:19-24``: u on x = 0.5 and v on y = 0.5 sampled at 49 points
+    by linear interpolation of the cell values, against Ghia's Re = 100 profiles."""
+    xc, U = centres(s)[:, :2], velocity(s)
+    y = np.linspace(0.02, 0.98, 49)
+    u = griddata(xc, U[:, 0], np.column_stack([np.full_like(y, 0.5), y]))
+    v = griddata(xc, U[:, 1], np.column_stack([y, np.full_like(y, 0.5)]))
+    yy, uu = np.concatenate([[0.0], y, [1.0]]), np.concatenate([[0.0], u, [1.0]])
+    xx, vv = np.concatenate([[0.0], y, [1.0]]), np.concatenate([[0.0], v, [0.0]])
+    du = np.interp(GHIA_Y, yy, uu) - GHIA_U100
+    dv = np.interp(GHIA_X, xx, vv) - GHIA_V100
+    return {"u_max_err": float(np.abs(du).max()), "v_max_err": float(np.abs(dv).max())}
+
+
+# ---------------------------------------------------------------- the four cases of test_unstructured.py
+def cavity_skewed_triangles(mod):
+    """``test_unstructured.py:27-34``: 32 x 32 jittered triangles, Re = 100."""
+    m = cartesian_mesh_2d(32, 32, triangulate=True, jitter=0.2)
+    return m, make_case(mod, {"top": dict(kind="wall", velocity=[1.0, 0.0])}, viscosity=0.01, max_iter=3000,
+                        tol=1e-6)
+
+
+def cavity_simple(mod):
+    """``test_unstructured.py:37-43``: SIMPLE on 24 x 24 quads (relax_p defaults to 0.3)."""
+    m = cartesian_mesh_2d(24, 24)
+    return m, make_case(mod, {"top": dict(kind="wall", velocity=[1.0, 0.0])}, viscosity=0.01, algorithm="SIMPLE",
+                        max_iter=4000, tol=1e-5)
+
+
+def heated_channel(mod):
+    """``test_unstructured.py:46-59``: 60 x 12 channel, unit normal inflow at T = 0, bottom wall
+    at T = 1."""
+    m = cartesian_mesh_2d(60, 12, 5.0, 1.0, names=("inlet", "outlet", "bottom", "top"),
+                          types=("velocity-inlet", "pressure-outlet", "wall", "wall"))
+    bnd = {"inlet": dict(kind="inlet", normal_speed=1.0, value=0.0), "outlet": dict(kind="outlet"),
+           "bottom": dict(kind="wall", thermal="temperature", value=1.0), "top": dict(kind="wall")}
+    return m, make_case(mod, bnd, viscosity=0.02, energy=True, conductivity=0.02, max_iter=1500, tol=1e-6)
+
+
+def extruded_cavity(mod):
+    """``test_unstructured.py:62-71``: the 16 x 16 cavity extruded to 2 layers in z."""
+    m = extrude_to_3d(cartesian_mesh_2d(16, 16), 2, 0.2)
+    return m, make_case(mod, {"top": dict(kind="wall", velocity=[1.0, 0.0, 0.0])}, viscosity=0.01, max_iter=400,
+                        tol=1e-6)
+
+
+def check_cavity_skewed_triangles(s):
+    assert s.converged
+    e = ghia_errors(s)
+    assert e["u_max_err"] < 0.03 and e["v_max_err"] < 0.03, e
+    return e
+
+
+def check_cavity_simple(s):
+    assert s.case.relax_p == 0.3
+    assert s.converged
+
+
+def check_heated_channel(s):
+    assert s.converged
+    mb = mass_balance(s)
+    assert abs(mb["inlet"] + 1.0) < 1e-9 and abs(sum(mb.values())) < 1e-8, mb
+    T = temperature(s)
+    assert -1e-8 <= T.min() and T.max() <= 1.0 + 1e-6
+
+
+def check_extruded_cavity(s):
+    U = velocity(s)
+    assert np.abs(U[:, 2]).max() < 1e-8
+    assert U[:, 0].min() < -0.15
+    assert s.residuals()["scaled"]["z-velocity"] < 1e-3
+
+
+PINS = {"cavity_skewed_triangles": (cavity_skewed_triangles, check_cavity_skewed_triangles),
+        "simple_algorithm": (cavity_simple, check_cavity_simple),
+        "channel_mass_balance_and_heat": (heated_channel, check_heated_channel),
+        "extruded_cavity_2d": (extruded_cavity, check_extruded_cavity)}

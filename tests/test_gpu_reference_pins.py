@@ -1,0 +1,68 @@
+"""The reference's physics tests of the SIMPLE/SIMPLEC path (``tests/test_unstructured.py:27-71``)
+run by the HIP solver, and the README cases of the reference's example meshes
+(``examples/README.md``: the binary 40 x 40 cavity at Re = 100 and the heated triangle duct),
+each also against the oracle's converged fields on the same mesh."""
+import numpy as np
+import pytest
+
+import oracle.simplec_oracle as O
+from gputest_amd import case as C
+from gputest_amd.fluent import read_any_mesh
+
+from fluent_cases import EXAMPLES, example_case
+from reference_pins import PINS, ghia_errors, mass_balance, run, solver, temperature, velocity
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def _against_oracle(s, o, energy=False):
+    """Both runs follow the same trajectory (the same iteration count) and stop at the same
+    scaled residual; their converged fields agree to round-off (measured 1e-13 or less)."""
+    errs = {"U": rel_l2(velocity(s), velocity(o))}
+    p = s.p.detach().cpu().numpy()
+    errs["p"] = rel_l2(p - p.mean(), o.p - o.p.mean())
+    if energy:
+        errs["T"] = rel_l2(temperature(s), temperature(o))
+    print("iterations", s.iteration, "oracle", o.iteration, "vs oracle", errs, flush=True)
+    for k, e in errs.items():
+        assert e < 1e-10, (k, e)
+    assert s.iteration == o.iteration
+
+
+@pytest.mark.parametrize("name", list(PINS))
+def test_hip_solver_meets_the_reference_pin(name):
+    build, check = PINS[name]
+    mesh, case = build(C)
+    s = run(solver(C, mesh, case))
+    print(name, "iterations", s.iteration, check(s), flush=True)
+    meshO, caseO = build(O)
+    _against_oracle(s, run(O.Simplec(meshO, caseO)), energy=case.energy)
+
+
+def test_example_cavity_binary_mesh_matches_ghia():
+    mesh = read_any_mesh(EXAMPLES["cavity_quad"])[0]
+    s = run(solver(C, mesh, example_case(C, mesh, "cavity_quad", 3000, 1e-6)))
+    e = ghia_errors(s)
+    print("cavity_quad", s.iteration, e, flush=True)
+    assert s.converged and e["u_max_err"] < 0.03 and e["v_max_err"] < 0.03
+    _against_oracle(s, run(O.Simplec(mesh, example_case(O, mesh, "cavity_quad", 3000, 1e-6))))
+
+
+def test_example_heated_duct():
+    mesh = read_any_mesh(EXAMPLES["heated_duct_tri"])[0]
+    s = run(solver(C, mesh, example_case(C, mesh, "heated_duct_tri", 3000, 1e-6)))
+    mb = mass_balance(s)
+    print("heated_duct_tri", s.iteration, mb, s.heat_flow(), flush=True)
+    assert s.converged
+    assert abs(mb["inlet"] + 2.0) < 1e-9 and abs(sum(mb.values())) < 1e-8
+    T = temperature(s)
+    assert -1e-8 <= T.min() and T.max() <= 1.0 + 1e-6
+    o = run(O.Simplec(mesh, example_case(O, mesh, "heated_duct_tri", 3000, 1e-6)))
+    _against_oracle(s, o, energy=True)
+    hd, ho = s.heat_flow(), o.heat_flow()
+    for z in ho:
+        assert abs(hd[z] - ho[z]) <= 1e-7 * max(1.0, abs(ho[z])), (z, hd[z], ho[z])
