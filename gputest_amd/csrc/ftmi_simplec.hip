@@ -281,7 +281,7 @@ struct AsmArgs {
     double gamma;
     const double* gam_f;     // per interior face diffusivity (nullptr: gamma everywhere)
     const double* gam_b;     // per boundary face diffusivity (nullptr: gamma)
-    const double* extra[3];  // per-cell source added to field q's source (nullptr: none)
+    const double* extra[3];  // per-cell source added to field q's source; energy: extra[0] (nullptr: none)
     int dir_mode;  // 0: momentum (walls, inlets, symmetry Dirichlet); 1: energy (fixed-T faces)
     const double* phi[3];
     const double* grad[3];
@@ -426,7 +426,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
                 if (A.extra[q]) src = src + (R)A.extra[q][c];
                 rhs = rhs + src;
             } else {
-                rhs = rhs + qin;
+                rhs = rhs + (A.extra[0] ? qin + (R)A.extra[0][c] : qin);  // energy source after the flux walls
             }
             const R Aphi = (kDiff ? ex_u : diag) * pc[q] + ax[q];
             acc[2 * q] += fabs(rhs - Aphi);

@@ -123,6 +123,8 @@ class SimpleSolver:
         self._has_lag = False
         self._has_lagT = False
         self._init_flux()
+        self.mom_src = None  # (dim, cells) volumetric momentum source times the volume
+        self.en_src = None  # (cells,) volumetric heat source times the volume over cp
         self.turb = None
         if case.turbulent:
             from .turbulence import DeviceSST
@@ -224,6 +226,29 @@ class SimpleSolver:
         nat.check(L.ftmi_init_flux(ctypes.byref(fv.c_mesh), ctypes.byref(self.c_bc), float(self.case.density),
                                    nat.ptr(self.U), nat.ptr(self.ub), nat.ptr(self.F), nat.ptr(self.Fb), s), "init_flux")
 
+    def set_sources(self, momentum=None, energy=None) -> None:
+        """Volumetric sources per unit volume (``simple.py:164-182``): momentum (cells, dim)
+        [force/volume] and energy (cells,) [heat/volume]. They are kept on the device times the
+        cell volume (energy also over cp) and added to the momentum source after the body force
+        and to the energy equation's heat input after the flux walls, as the reference adds
+        them. ``None`` removes a source."""
+        vol = np.asarray(self.fv.vol_h, np.float64)
+        if momentum is not None:
+            if self.turb is not None:
+                raise NotImplementedError("volumetric momentum sources with the SST model are not built")
+            m = np.asarray(momentum, np.float64).reshape(self.nc, self.dim) * vol[:, None]
+            self.mom_src = torch.as_tensor(np.ascontiguousarray(m.T), device=self.device)
+        else:
+            self.mom_src = None
+        if energy is not None:
+            if not self.energy:
+                raise ValueError("energy source given but the energy equation is off")
+            e = np.asarray(energy, np.float64).reshape(self.nc) * vol / self.case.cp
+            self.en_src = torch.as_tensor(e, device=self.device)
+        else:
+            self.en_src = None
+        self._graph = None  # a recorded graph holds the previous source pointers
+
     # ------------------------------------------------------------------ the outer iteration
     @property
     def finished(self) -> bool:
@@ -323,6 +348,9 @@ class SimpleSolver:
             a.gam_f, a.gam_b = nat.ptr(gm_f), nat.ptr(gm_b)
             for k in range(dim):
                 a.extra[k] = nat.ptr(extra[k])
+        elif self.mom_src is not None:  # simple.py:535-536
+            for k in range(dim):
+                a.extra[k] = nat.ptr(self.mom_src[k])
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "momentum assembly")
         infos = []
         if self.soff is not None:
@@ -460,6 +488,8 @@ class SimpleSolver:
         if self.turb is not None:  # k/cp + mu_t / Pr_t (simple.py:781-782)
             ge_f, ge_b = self.turb.energy_terms(L, M, B, s, c.conductivity / c.cp)
             a.gam_f, a.gam_b = nat.ptr(ge_f), nat.ptr(ge_b)
+        if self.en_src is not None:  # simple.py:795-796
+            a.extra[0] = nat.ptr(self.en_src)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
         if self.mixed:
             info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,

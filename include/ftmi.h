@@ -308,8 +308,11 @@ typedef struct ftmi_assemble_args {
     /* face-varying diffusivity (simple.py:333-338, the (interior, boundary) form of gamma):
      * gam_f (nf) and gam_b (nb), or NULL for `gamma` everywhere */
     const double *gam_f, *gam_b;
-    const double* extra[3];   /* momentum: a further per-cell source of field q (the turbulent
-                                 transpose stress, simple.py:546-548), or NULL */
+    const double* extra[3];   /* momentum: a further per-cell source of field q, added after
+                                 the body force (the volumetric source of simple.py:535-536, or
+                                 the turbulent transpose stress of simple.py:546-548); energy:
+                                 extra[0] is added to the heat input after the flux walls
+                                 (simple.py:795-796); NULL: none */
 } ftmi_assemble_args;
 
 

@@ -504,6 +504,8 @@ class Simplec:
         self.T = np.full(self.nc, case.init_T) if self.energy else None
         self.F = case.density * np.sum(fv.face_interp_vec(self.U) * fv.S, axis=1)
         self.Fb = self._fixed_flux(self._ubvals(self.U))
+        self.mom_source = None  # (nc, dim) force per cell (already times the volume)
+        self.energy_source = None  # (nc,) heat per cell (times the volume, over cp)
         self.turb = None
         if case.turbulence == "k-omega-sst":
             from oracle.turbulence_oracle import SST
@@ -626,6 +628,8 @@ class Simplec:
                                                                      + [0.0] * 3)[:dim], float)
         if np.any(drive):
             src = src + fv.vol[:, None] * drive[None, :]
+        if self.mom_source is not None:  # simple.py:535-536
+            src = src + self.mom_source
         gm = mu
         if self.turb is not None:  # mu + mu_t on faces, the wall-function viscosity on walls
             gm = self.turb.momentum_viscosity(U)
@@ -732,6 +736,8 @@ class Simplec:
         gT = fv.gradient(T, Tb)
         self._gT = gT
         q_in = fv.Cb @ (flux * self.tval * fv.Sb_mag / c.cp)
+        if self.energy_source is not None:  # simple.py:795-796
+            q_in = q_in + self.energy_source
         diag, aPN, aNP, Db = self._coefficients(gam, fixed.astype(np.float64))
         rhs = self._rhs(gam, T, gT, Tb, Db, q_in)
         A = fv.matrix(diag, aPN, aNP)
@@ -794,6 +800,20 @@ class Simplec:
         return False
 
     # --- state and reports (simple.py:822-833, 970-1072) ---
+    def set_sources(self, momentum=None, energy=None):
+        """Volumetric sources per unit volume, momentum (nc, dim) and energy (nc,), stored times
+        the cell volume (energy also over cp) as ``simple.py:164-182`` does."""
+        if momentum is not None:
+            self.mom_source = np.asarray(momentum, np.float64).reshape(self.fv.nc, self.fv.dim) * self.fv.vol[:, None]
+        else:
+            self.mom_source = None
+        if energy is not None:
+            if not self.energy:
+                raise ValueError("energy source given but the energy equation is off")
+            self.energy_source = np.asarray(energy, np.float64).reshape(self.fv.nc) * self.fv.vol / self.case.cp
+        else:
+            self.energy_source = None
+
     def residuals(self):
         r = dict(getattr(self, "_last_res", {}))
         return {"raw": dict(getattr(self, "_last_raw", r)), "scaled": r}
