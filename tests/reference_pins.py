@@ -1,4 +1,5 @@
-"""The reference's physics pins of the SIMPLE/SIMPLEC path (``tests/test_unstructured.py:27-71``)
+"""The reference's physics pins of the SIMPLE/SIMPLEC path (``tests/test_unstructured.py:27-71``,
+the periodic Poiseuille flow of ``tests/test_zones.py:24-37``)
 and the cases of its example meshes (``examples/README.md``), written once for the oracle
 (``mod`` = ``oracle.simplec_oracle``) and for the device solver (``mod`` = ``gputest_amd.case``).
 
@@ -146,7 +147,36 @@ def check_extruded_cavity(s):
     assert s.residuals()["scaled"]["z-velocity"] < 1e-3
 
 
+def periodic_poiseuille(mod, tri):
+    """``tests/test_zones.py:24-37``: a 4 x 12 strip made periodic in x, written as a Fluent
+    mesh and read back (the periodic pair survives), driven by a unit body force at nu = 0.1."""
+    import os
+    import tempfile
+
+    from gputest_amd.fluent import read_fluent_mesh, write_fluent_mesh
+    from gputest_amd.mesh import make_periodic
+
+    m = make_periodic(cartesian_mesh_2d(4, 12, 0.5, 1.0, names=("in", "out", "bottom", "top"), triangulate=tri),
+                      "in", "out")
+    assert [z.name for z in m.boundary_zones()] == ["bottom", "top"]
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "per.msh")
+        write_fluent_mesh(m, path)
+        m2 = read_fluent_mesh(path)
+    assert len(m2.periodic) == 1 and np.allclose(m2.periodic[0]["translation"], [0.5, 0.0])
+    return m2, make_case(mod, {}, viscosity=0.1, body_force=[1.0, 0.0, 0.0], max_iter=6000, tol=1e-9, relax_u=0.9)
+
+
+def check_periodic_poiseuille(s):
+    assert s.converged
+    xc, U = centres(s), velocity(s)
+    assert np.abs(U[:, 0] - 5 * xc[:, 1] * (1 - xc[:, 1])).max() < 0.02
+    assert np.abs(U[:, 1]).max() < 1e-8
+
+
 PINS = {"cavity_skewed_triangles": (cavity_skewed_triangles, check_cavity_skewed_triangles),
         "simple_algorithm": (cavity_simple, check_cavity_simple),
         "channel_mass_balance_and_heat": (heated_channel, check_heated_channel),
-        "extruded_cavity_2d": (extruded_cavity, check_extruded_cavity)}
+        "extruded_cavity_2d": (extruded_cavity, check_extruded_cavity),
+        "periodic_poiseuille_quads": (lambda mod: periodic_poiseuille(mod, False), check_periodic_poiseuille),
+        "periodic_poiseuille_triangles": (lambda mod: periodic_poiseuille(mod, True), check_periodic_poiseuille)}
