@@ -19,15 +19,29 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
 
 
-def _against_oracle(s, o, energy=False):
+def _against_oracle(s, o, energy=False, periodic=False):
     """Both runs follow the same trajectory (the same iteration count) and stop at the same
-    scaled residual; their converged fields agree to round-off (measured 1e-13 or less)."""
+    scaled residual; their converged fields agree to round-off (measured 1e-13 or less).
+
+    A periodic channel driven by a body force has a singular pressure system whose solution is
+    a constant: its mean-free pressure is round-off noise, which also moves the trajectory
+    (``test_gpu_parity.py`` bounds the same effect on a short periodic run). There both runs
+    stop at the scaled residual 1e-9 on different trajectories, so the velocities agree to
+    1e-7 and the noise pressures stay below 1e-9 of the dynamic pressure u_max^2 (the oracle's:
+    3e-13 on quads, 1.4e-12 on triangles)."""
     errs = {"U": rel_l2(velocity(s), velocity(o))}
     p = s.p.detach().cpu().numpy()
-    errs["p"] = rel_l2(p - p.mean(), o.p - o.p.mean())
+    if periodic:
+        umax = float(np.abs(velocity(o)).max())
+        errs["p"] = float(max(np.abs(p - p.mean()).max(), np.abs(o.p - o.p.mean()).max())) / umax ** 2
+    else:
+        errs["p"] = rel_l2(p - p.mean(), o.p - o.p.mean())
     if energy:
         errs["T"] = rel_l2(temperature(s), temperature(o))
     print("iterations", s.iteration, "oracle", o.iteration, "vs oracle", errs, flush=True)
+    if periodic:
+        assert errs["U"] < 1e-7 and errs["p"] < 1e-9, errs
+        return
     for k, e in errs.items():
         assert e < 1e-10, (k, e)
     assert s.iteration == o.iteration
@@ -40,7 +54,7 @@ def test_hip_solver_meets_the_reference_pin(name):
     s = run(solver(C, mesh, case))
     print(name, "iterations", s.iteration, check(s), flush=True)
     meshO, caseO = build(O)
-    _against_oracle(s, run(O.Simplec(meshO, caseO)), energy=case.energy)
+    _against_oracle(s, run(O.Simplec(meshO, caseO)), energy=case.energy, periodic=name.startswith("periodic"))
 
 
 def test_example_cavity_binary_mesh_matches_ghia():
