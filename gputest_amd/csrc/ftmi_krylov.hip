@@ -691,12 +691,18 @@ __device__ inline T seg_sum(T v) {
     return v;
 }
 
+// sum_j a_ij f(j) over the entries lane, lane + TPR, ... of row i, in that order
+template <int TPR, typename T, class F>
+__device__ inline T lane_sum(const CsrOp<T>& A, int64_t i, int lane, F f) {
+    const int32_t p = A.ptr[i] + lane, e = A.ptr[i + 1];
+    const int d = p < e ? (e - p + TPR - 1) / TPR : 0;
+    return gather_sum(
+        T(0), d, 0, [&](int k) { return A.col[p + TPR * k]; }, [&](int k) { return A.val[p + TPR * k]; }, f);
+}
+
 template <int TPR, typename T>
 __device__ inline T row_dot(const CsrOp<T>& A, int64_t i, const T* __restrict__ x, int lane) {
-    T s = T(0);
-    const int32_t e = A.ptr[i + 1];
-    for (int32_t j = A.ptr[i] + lane; j < e; j += TPR) s += A.val[j] * x[A.col[j]];
-    return seg_sum<TPR>(s);
+    return seg_sum<TPR>(lane_sum<TPR>(A, i, lane, [&](int32_t j) { return x[j]; }));
 }
 
 inline int grid_rows(int64_t n, int tpr) { return grid_for(n * tpr); }
@@ -710,12 +716,7 @@ __global__ void __launch_bounds__(kBlock) vc_pre2(int64_t n, CsrOp<T> A, const T
     const T w = (T)w_d;
     RowGroups<TPR> G;
     for (int64_t i = G.first; i < n; i += G.stride) {
-        T ax = T(0);
-        const int32_t e = A.ptr[i + 1];
-        for (int32_t j = A.ptr[i] + G.lane; j < e; j += TPR) {
-            const int32_t c = A.col[j];
-            ax += A.val[j] * ((w * Dinv[c]) * b[c]);
-        }
+        T ax = lane_sum<TPR>(A, i, G.lane, [&](int32_t c) { return (w * Dinv[c]) * b[c]; });
         ax = seg_sum<TPR>(ax);
         if (G.lane == 0) {
             const T wd = w * Dinv[i];

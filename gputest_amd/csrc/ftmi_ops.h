@@ -5,6 +5,37 @@
 
 namespace ftmi {
 
+// s + sum_{k<d} coef(k) * f(idx(k)), added in k order. The loop takes kGather entries at a time:
+// first their indices and coefficients, then their gathered values, then the sums, so the loads
+// of a chunk are in flight together. A loop over one entry at a time waits out two dependent
+// memory round trips per entry (index, then value), which makes the short rows of these
+// operators latency-bound. The slots of a chunk past d gather f(safe) (safe: any index f may
+// read) and are not added, so the arithmetic and its order are those of the one-at-a-time loop.
+constexpr int kGather = 4;
+
+template <typename T, class IDX, class COEF, class F>
+__device__ inline T gather_sum(T s, int d, int32_t safe, IDX idx, COEF coef, F f) {
+    for (int k0 = 0; k0 < d; k0 += kGather) {
+        int32_t nb[kGather];
+        T a[kGather], v[kGather];
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) {
+            nb[u] = safe;
+            a[u] = T(0);
+            if (k0 + u < d) {
+                nb[u] = idx(k0 + u);
+                a[u] = coef(k0 + u);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kGather; ++u) v[u] = f(nb[u]);
+#pragma unroll
+        for (int u = 0; u < kGather; ++u)
+            if (k0 + u < d) s += a[u] * v[u];
+    }
+    return s;
+}
+
 // Plain CSR (indptr, indices, data): y_i = sum_j a_ij x_j in column order.
 template <typename T>
 struct CsrOp {
@@ -12,18 +43,14 @@ struct CsrOp {
     const int32_t* col;
     const T* val;
     __device__ inline T row(int64_t i, const T* __restrict__ x) const {
-        T s = T(0);
-        const int32_t e = ptr[i + 1];
-        for (int32_t j = ptr[i]; j < e; ++j) s += val[j] * x[col[j]];
-        return s;
+        return row_f(i, [&](int32_t j) { return x[j]; });
     }
     // sum_j a_ij f(j), with f giving x_j on the fly
     template <class F>
     __device__ inline T row_f(int64_t i, F f) const {
-        T s = T(0);
-        const int32_t e = ptr[i + 1];
-        for (int32_t j = ptr[i]; j < e; ++j) s += val[j] * f(col[j]);
-        return s;
+        const int32_t p = ptr[i];
+        return gather_sum(
+            T(0), ptr[i + 1] - p, 0, [&](int k) { return col[p + k]; }, [&](int k) { return val[p + k]; }, f);
     }
 };
 
@@ -37,17 +64,14 @@ struct LduOp {
     const int32_t* nbr;
     const T* off;
     __device__ inline T row(int64_t i, const T* __restrict__ x) const {
-        T s = diag[i] * x[i];
-        const int32_t e = ptr[i + 1];
-        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * x[nbr[k]];
-        return s;
+        return row_f(i, [&](int32_t j) { return x[j]; });
     }
     template <class F>
     __device__ inline T row_f(int64_t i, F f) const {
-        T s = diag[i] * f(i);
-        const int32_t e = ptr[i + 1];
-        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * f(nbr[k]);
-        return s;
+        const int32_t p = ptr[i];
+        return gather_sum(
+            diag[i] * f(i), ptr[i + 1] - p, (int32_t)i, [&](int k) { return nbr[p + k]; },
+            [&](int k) { return off[p + k]; }, f);
     }
 };
 
@@ -63,19 +87,14 @@ struct LduSellOp {
     const int32_t* nbr;
     const T* off;
     __device__ inline T row(int64_t i, const T* __restrict__ x) const {
-        T s = diag[i] * x[i];
-        const int32_t b = base[i >> 6] + (int32_t)(i & 63);
-        const int d = deg[i];
-        for (int k = 0; k < d; ++k) s += off[b + 64 * k] * x[nbr[b + 64 * k]];
-        return s;
+        return row_f(i, [&](int32_t j) { return x[j]; });
     }
     template <class F>
     __device__ inline T row_f(int64_t i, F f) const {
-        T s = diag[i] * f(i);
         const int32_t b = base[i >> 6] + (int32_t)(i & 63);
-        const int d = deg[i];
-        for (int k = 0; k < d; ++k) s += off[b + 64 * k] * f(nbr[b + 64 * k]);
-        return s;
+        return gather_sum(
+            diag[i] * f(i), (int)deg[i], (int32_t)i, [&](int k) { return nbr[b + 64 * k]; },
+            [&](int k) { return off[b + 64 * k]; }, f);
     }
 };
 
@@ -90,19 +109,16 @@ struct LduDiffOp {
     const int32_t* nbr;
     const T* off;
     __device__ inline T row(int64_t i, const T* __restrict__ x) const {
-        const T xi = x[i];
-        T s = ex[i] * xi;
-        const int32_t e = ptr[i + 1];
-        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * (x[nbr[k]] - xi);
-        return s;
+        return row_f(i, [&](int32_t j) { return x[j]; });
     }
     template <class F>
     __device__ inline T row_f(int64_t i, F f) const {
         const T xi = f(i);
-        T s = ex[i] * xi;
-        const int32_t e = ptr[i + 1];
-        for (int32_t k = ptr[i]; k < e; ++k) s += off[k] * (f(nbr[k]) - xi);
-        return s;
+        const int32_t p = ptr[i];
+        return gather_sum(
+            ex[i] * xi, ptr[i + 1] - p, (int32_t)i, [&](int k) { return nbr[p + k]; },
+            [&](int k) { return off[p + k]; },
+            [&](int32_t j) { return f(j) - xi; });
     }
 };
 

@@ -175,3 +175,28 @@ def test_sst_graph_replays_equal_eager_iterations():
     for name in ("k", "w", "mut"):
         assert torch.equal(getattr(eager.turb, name), getattr(graph.turb, name)), name
     assert eager.residuals()["raw"] == graph.residuals()["raw"]
+
+
+def test_turbulent_channel_runs_and_reports_fields():
+    """``tests/test_turbulence.py:34-50`` for the SST model on the device: 150 iterations of a
+    20 x 8 channel at nu = 1e-4; finite velocity, mu_t / mu above 1, positive k, y+ reported,
+    and a solver loaded from the state dictionary holds the same k."""
+    from gputest_amd.mesh import cartesian_mesh_2d
+
+    m = cartesian_mesh_2d(20, 8, 4.0, 1.0, names=("in", "out", "bottom", "top"),
+                          types=("velocity-inlet", "pressure-outlet", "wall", "wall"))
+    c = C.UnstructuredCase(boundaries={"in": C.BoundarySpec("inlet", normal_speed=1.0)}, viscosity=1e-4,
+                           turbulence="k-omega-sst", max_iter=150, tol=1e-12)
+    s = SimpleSolver(m, c, DEV)
+    for _ in range(150):
+        s.step()
+    f = s.fields()
+    print("mu_t/mu max", f["viscosity-ratio"].max(), "k min", f["turb-kinetic-energy"].min(), "y+ max",
+          f["y-plus"].max(), "guard", s.guard_events)
+    assert np.all(np.isfinite(f["x-velocity"])) and f["viscosity-ratio"].max() > 1.0
+    assert f["turb-kinetic-energy"].min() > 0
+    assert "y-plus" in f and f["y-plus"].max() > 0
+    st = s.state_dict()
+    s2 = SimpleSolver(m, c, DEV)
+    s2.load_state(st)
+    assert np.allclose(h(s2.turb.k), st["k"])

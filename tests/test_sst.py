@@ -76,3 +76,13 @@ def test_oracle_reproduces_the_reference_channel_validation():
     assert s.converged and not s.guard_events
     assert abs(float(tau.mean()) - 1.0) < 0.01
     assert np.abs(err[DNS_YPLUS >= 1.0]).max() < 5.0, err
+
+
+def test_unknown_model_rejected():
+    """``tests/test_turbulence.py:70-72``, for the device case and the oracle's."""
+    from gputest_amd.case import UnstructuredCase
+
+    with pytest.raises(ValueError):
+        UnstructuredCase(turbulence="spalart-allmaras-typo")
+    with pytest.raises(ValueError):
+        O.Case(turbulence="spalart-allmaras-typo")
