@@ -126,6 +126,25 @@ def pressure_solve_roofline(s, info) -> dict:
             "note": "includes each solve's start-up and the host polls; the AMG set-up is excluded"}
 
 
+def copy_ceiling(nbytes=256 << 20, reps=10):
+    """Rate of a plain device-to-device copy (the HIP runtime's copy kernel), read plus write
+    bytes over the event time of `reps` copies: the practical HBM ceiling against which the
+    kernels' fractions of the 8 TB/s peak can be read."""
+    import torch
+
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * nbytes * reps / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    return {"gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_copy": 2 * nbytes}
+
+
 def calibration_copy(nbytes=256 << 20):
     """One device-to-device copy of known size: the reference dispatch against which the
     FETCH_SIZE / WRITE_SIZE counters of a profiled run are scaled."""
@@ -339,6 +358,7 @@ def main():
         del sm
     pres = pressure_solve_roofline(s, info)
     roof = measure_roofline(s)
+    roof["copy_ceiling"] = copy_ceiling()
     calibration_copy()
     out = {"metric": "SIMPLEC outer it/s at 1M cells (FP64)", "value": round(args.steps / dt, 3), "unit": "it/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
