@@ -126,6 +126,20 @@ def pressure_solve_roofline(s, info) -> dict:
             "note": "includes each solve's start-up and the host polls; the AMG set-up is excluded"}
 
 
+def window_difference(a, b) -> dict:
+    """Relative L2 difference of the fields of two solvers after the same outer iterations from
+    the same start (pressure mean-free): how far the mixed-precision trajectory is from FP64's."""
+    import torch
+
+    def rel(x, y):
+        return float(torch.linalg.vector_norm(x - y) / torch.clamp(torch.linalg.vector_norm(y), min=1e-300))
+
+    out = {"U": rel(a.U, b.U), "p": rel(a.p - a.p.mean(), b.p - b.p.mean())}
+    if a.energy:
+        out["T"] = rel(a.T, b.T)
+    return {k: float(f"{v:.3g}") for k, v in out.items()}
+
+
 def copy_ceiling(nbytes=256 << 20, reps=10):
     """Rate of a plain device-to-device copy (the HIP runtime's copy kernel), read plus write
     bytes over the event time of `reps` copies: the practical HBM ceiling against which the
@@ -354,7 +368,8 @@ def main():
         _, mcase, _ = _workload(args.nx, args.ny, "mixed")
         sm, dtm, infom = run_timed(SimpleSolver, mesh, mcase, args.warmup, args.steps, "mixed")
         mixed = {"value": round(args.steps / dtm, 3), "ms_per_step": round(dtm / args.steps * 1e3, 3),
-                 "speedup_vs_fp64": round(dt / dtm, 3), "dtype": "f32 solves + f64 refinement", **infom}
+                 "speedup_vs_fp64": round(dt / dtm, 3), "dtype": "f32 solves + f64 refinement",
+                 "rel_l2_vs_fp64": window_difference(sm, s), **infom}
         del sm
     pres = pressure_solve_roofline(s, info)
     roof = measure_roofline(s)
