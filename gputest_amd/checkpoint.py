@@ -2,7 +2,7 @@
 
 The file format is the reference's: one ``<tag>.npz`` per checkpoint holding the arrays of
 ``SimpleSolver.state_dict()`` (``U`` as (cells, dim), ``p``, ``F``, ``Fb``, ``T`` when the energy
-equation is on, and the scalars ``iteration``, ``time``, ``time_step``,
+equation is on, ``k``, ``w`` and ``mut`` with the SST model, and the scalars ``iteration``, ``time``, ``time_step``,
 ``res_continuity_norm``) plus a ``__meta__`` entry, the UTF-8 bytes of a JSON object. It is
 written to a temporary name and renamed, so a reader never sees half a file, and only the
 newest ``keep`` files of the directory are kept.
@@ -28,6 +28,7 @@ import numpy as np
 import torch
 
 _STATE_ARRAYS = ("U", "p", "F", "Fb", "T")
+_TURB_ARRAYS = ("k", "w", "mut")
 
 
 class Snapshot:
@@ -52,12 +53,22 @@ class Snapshot:
         return st
 
 
+def _device_state(solver) -> Dict[str, torch.Tensor]:
+    """The device arrays of ``state_dict()``: the flow fields, T with the energy equation, and
+    the SST fields k, w, mut with turbulence (``simple.py:1033-1046`` of the reference; the
+    turbulence model's ``state()``)."""
+    out = {k: getattr(solver, k) for k in _STATE_ARRAYS if k != "T" or solver.energy}
+    turb = getattr(solver, "turb", None)
+    if turb is not None:
+        out.update({k: getattr(turb, k) for k in _TURB_ARRAYS})
+    return out
+
+
 class _Staging:
     """Pinned host buffers for one snapshot of a given solver layout, and whether they are free."""
 
     def __init__(self, solver):
-        self.bufs = {k: torch.empty(getattr(solver, k).shape, dtype=getattr(solver, k).dtype, pin_memory=True)
-                     for k in _STATE_ARRAYS if k != "T" or solver.energy}
+        self.bufs = {k: torch.empty(v.shape, dtype=v.dtype, pin_memory=True) for k, v in _device_state(solver).items()}
         self.free = threading.Event()
         self.free.set()
 
@@ -67,8 +78,12 @@ def snapshot(solver, staging: Optional[_Staging] = None) -> Snapshot:
     staging = staging or _Staging(solver)
     staging.free.wait()
     staging.free.clear()
+    src = _device_state(solver)
+    if set(src) != set(staging.bufs):
+        staging.free.set()
+        raise ValueError(f"checkpoint staging holds {sorted(staging.bufs)}, the solver has {sorted(src)}")
     for k, buf in staging.bufs.items():
-        buf.copy_(getattr(solver, k), non_blocking=True)
+        buf.copy_(src[k], non_blocking=True)
     ev = torch.cuda.Event()
     ev.record()
     scalars = {"iteration": int(solver.iteration), "time": 0.0, "time_step": 0,

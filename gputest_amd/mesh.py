@@ -233,29 +233,54 @@ def sort_faces_by_zone(mesh: UnstructuredMesh) -> UnstructuredMesh:
                             periodic=periodic)
 
 
+def _land_on(src: np.ndarray, dst: np.ndarray, reach: float):
+    """For every point of ``src``, the nearest point of ``dst`` no farther than ``reach``, found by
+    a sort-and-sweep along the coordinate in which ``dst`` spreads most: the candidates of a point
+    are the ``dst`` points whose coordinate there lies within ``reach`` of its own. Returns
+    (index, distance), with index -1 and distance inf where no candidate is in reach."""
+    axis = int(np.argmax(np.ptp(dst, axis=0))) if dst.size else 0
+    order = np.argsort(dst[:, axis], kind="stable")
+    key = dst[order, axis]
+    lo = np.searchsorted(key, src[:, axis] - reach, side="left")
+    width = np.searchsorted(key, src[:, axis] + reach, side="right") - lo
+    hit = np.full(src.shape[0], -1, dtype=np.int64)
+    dist = np.full(src.shape[0], np.inf)
+    for k in range(int(width.max(initial=0))):
+        rows = np.flatnonzero(width > k)
+        cand = order[lo[rows] + k]
+        d = np.sqrt(((dst[cand] - src[rows]) ** 2).sum(axis=1))
+        closer = d < dist[rows]
+        hit[rows[closer]], dist[rows[closer]] = cand[closer], d[closer]
+    return hit, dist
+
+
 def make_periodic(mesh: UnstructuredMesh, zone_a: str, zone_b: str, translation=None,
                   tol: float = 1e-6) -> UnstructuredMesh:
     """Pair two boundary zones as translational periodic, ``zone_b = zone_a + T``
-    (``unstructured.py:397-423``): faces are matched by centre after the shift, within
-    ``tol`` times the face size. Modifies and returns ``mesh``."""
-    from scipy.spatial import cKDTree
-
+    (``unstructured.py:397-423``). Each face of ``zone_a``, shifted by T, must land within
+    ``tol`` x its size (length in 2-D, square root of its area in 3-D) of a distinct face of
+    ``zone_b``; T defaults to the difference of the zones' mean face centres. The zones become
+    ``periodic`` / ``periodic-shadow`` and the pair is recorded in ``mesh.periodic`` with the
+    partner faces in ``zone_a``'s order (replacing any earlier pair of these zones). Raises
+    ValueError for unknown zones, unequal face counts or faces that do not match. Modifies and
+    returns ``mesh``."""
     za, zb = mesh.zone_by_name(zone_a), mesh.zone_by_name(zone_b)
     if za is None or zb is None:
-        raise ValueError(f"unknown zone(s): {zone_a!r}, {zone_b!r}")
+        raise ValueError(f"no zone named {zone_a!r} or {zone_b!r} in the mesh")
     fa, fb = mesh.faces_of_zone(za.id), mesh.faces_of_zone(zb.id)
     if fa.size != fb.size:
-        raise ValueError(f"periodic zones differ in size ({fa.size} vs {fb.size})")
-    fc, S = face_geometry(mesh)
-    T = np.asarray(translation, dtype=float) if translation is not None else fc[fb].mean(0) - fc[fa].mean(0)
-    dist, idx = cKDTree(fc[fb]).query(fc[fa] + T)
-    size = np.linalg.norm(S[fa], axis=1) ** (1.0 / max(mesh.dim - 1, 1))
-    if np.any(dist > tol * np.maximum(size, 1e-300) + 1e-12) or np.unique(idx).size != idx.size:
-        raise ValueError(f"faces of {zone_a!r} and {zone_b!r} do not match under translation {T}")
+        raise ValueError(f"zones {zone_a!r} and {zone_b!r} cannot be periodic: {fa.size} vs {fb.size} faces")
+    centre, area = face_geometry(mesh)
+    ca, cb = centre[fa], centre[fb]
+    shift = np.asarray(translation, dtype=float) if translation is not None else cb.mean(axis=0) - ca.mean(axis=0)
+    size = np.sqrt((area[fa] ** 2).sum(axis=1)) ** (1.0 / max(mesh.dim - 1, 1))
+    allowed = tol * np.maximum(size, 1e-300) + 1e-12
+    hit, dist = _land_on(ca + shift, cb, float(allowed.max(initial=0.0)))
+    if np.any(dist > allowed) or np.unique(hit).size != hit.size:
+        raise ValueError(f"zone {zone_b!r} is not zone {zone_a!r} shifted by {shift}: faces do not pair up")
     za.type, zb.type = "periodic", "periodic-shadow"
-    keep = [p for p in mesh.periodic if p["zone_a"] not in (za.id, zb.id)]
-    mesh.periodic = keep + [{"zone_a": za.id, "zone_b": zb.id, "faces_a": fa, "faces_b": fb[idx],
-                             "translation": T}]
+    pair = {"zone_a": za.id, "zone_b": zb.id, "faces_a": fa, "faces_b": fb[hit], "translation": shift}
+    mesh.periodic = [q for q in mesh.periodic if q["zone_a"] not in (za.id, zb.id)] + [pair]
     return mesh
 
 

@@ -28,7 +28,6 @@ system meets the tolerance; steps 1-2, 5 and 7 and all state stay FP64.
 from __future__ import annotations
 
 import ctypes
-import math
 import time
 from collections import defaultdict
 from typing import Dict
@@ -40,6 +39,7 @@ from . import _native as nat
 from .amg import SmoothedAggregationAMG
 from .case import FLUENT_TO_KIND, BoundarySpec, UnstructuredCase
 from .fvmesh import FVMesh
+from .guard import Settings, StabilityGuard
 from .krylov import KrylovInfo, Workspace, bicgstab_ldu, bicgstab_ldu_mixed, pcg_ldu, pcg_ldu_mixed, to_f32
 from .mesh import UnstructuredMesh
 
@@ -62,14 +62,6 @@ def auto_relax_p(non_orthogonality_deg: float) -> float:
 
 
 class SimpleSolver:
-    GUARD_GROWTH = 20.0
-    GUARD_ROUNDOFF = 1e-6
-    GUARD_EVERY = 20
-    GUARD_FACTOR = 0.6
-    GUARD_MIN_RELAX = 0.1
-    GUARD_FALLBACK = True
-    GUARD_MIN_RELAX_U = 0.2
-
     def __init__(self, mesh: UnstructuredMesh, case: UnstructuredCase, device="cuda"):
         t0 = time.perf_counter()
         self.case = case
@@ -536,12 +528,19 @@ class SimpleSolver:
 
     # ------------------------------------------------------------------ stability guard (simple.py:705-776)
     def _reset_guard(self):
-        self.guard_events = []
+        """A fresh guard (``guard.StabilityGuard``) with the case's algorithm and momentum
+        relaxation, holding a snapshot of the current state."""
         self.algo = self.case.algorithm
         self.relax_u = self.case.relax_u
-        self.guard_exhausted_at = None
-        self._guard_min = float("inf")
-        self._guard_snap = (self._device_snapshot(), float("inf")) if self.guard_enabled else None
+        self.guard = StabilityGuard(self.guard_enabled, self._device_snapshot)
+
+    @property
+    def guard_events(self):
+        return self.guard.events
+
+    @property
+    def guard_exhausted_at(self):
+        return self.guard.exhausted_at
 
     def _device_snapshot(self):
         st = {"U": self.U.clone(), "p": self.p.clone(), "F": self.F.clone(), "Fb": self.Fb.clone(),
@@ -567,45 +566,20 @@ class SimpleSolver:
             self._res_max5["continuity"] = float(st["res_continuity_norm"])
 
     def _stability_guard(self, raw_continuity: float, flux_scale: float) -> bool:
-        if not self.guard_enabled:
+        """Ask the guard about the iteration that just ended; on a rollback restore its snapshot
+        (the iteration counter keeps counting), take the new relaxation and force an AMG
+        rebuild. Returns True on a rollback."""
+        self.guard.enabled = self.guard_enabled
+        now = Settings(self.algo, self.relax_u, self.relax_p)
+        rb = self.guard.observe(self.iteration, raw_continuity, flux_scale, now)
+        if rb is None:
             return False
-        r = raw_continuity
-        ref = max(self._guard_min, self.GUARD_ROUNDOFF * flux_scale)
-        bad = not math.isfinite(r) or (self.iteration > 10 and r > self.GUARD_GROWTH * ref)
-        if bad and self._guard_snap is not None and self.relax_p * self.GUARD_FACTOR >= self.GUARD_MIN_RELAX:
-            snap, snap_r = self._guard_snap
-            at = self.iteration
-            self._restore_snapshot(snap)
-            self.iteration = self.step_count = at
-            self.relax_p *= self.GUARD_FACTOR
-            self._amg_age = 10 ** 9
-            self._guard_min = snap_r
-            self.guard_events.append({"detected_at": at, "restored_to": int(snap["iteration"]),
-                                      "relax_p": round(self.relax_p, 4), "continuity": r})
-            return True
-        stage2 = self.GUARD_FALLBACK and (self.algo == "SIMPLEC" or self.relax_u * 0.7 >= self.GUARD_MIN_RELAX_U - 1e-12)
-        if bad and self._guard_snap is not None and stage2:
-            snap, snap_r = self._guard_snap
-            at = self.iteration
-            self._restore_snapshot(snap)
-            self.iteration = self.step_count = at
-            if self.algo == "SIMPLEC":
-                self.algo, self.relax_u, self.relax_p = "SIMPLE", min(self.relax_u, 0.3), 0.3
-            else:
-                self.relax_u *= 0.7
-            self._amg_age = 10 ** 9
-            self._guard_min = snap_r
-            self.guard_events.append({"detected_at": at, "restored_to": int(snap["iteration"]), "algorithm": self.algo,
-                                      "relax_p": round(self.relax_p, 4), "relax_u": round(self.relax_u, 4),
-                                      "continuity": r})
-            return True
-        if bad and self.guard_exhausted_at is None:
-            self.guard_exhausted_at = self.iteration
-        if math.isfinite(r) and self.iteration > 5:
-            self._guard_min = min(self._guard_min, r)
-            if self.iteration % self.GUARD_EVERY == 0 and r <= 2.0 * self._guard_min:
-                self._guard_snap = (self._device_snapshot(), r)
-        return False
+        at = self.iteration
+        self._restore_snapshot(rb.snapshot)
+        self.iteration = self.step_count = at
+        self.algo, self.relax_u, self.relax_p = rb.settings.algorithm, rb.settings.relax_u, rb.settings.relax_p
+        self._amg_age = 10 ** 9
+        return True
 
     # ------------------------------------------------------------------ state, fields, reports
     def diagnostics(self) -> dict:

@@ -775,7 +775,8 @@ class Simplec:
             self.relax_p *= self.GUARD_FACTOR
             self._amg_age = 10 ** 9
             self._guard_min = snap_r
-            self.guard_events.append({"detected_at": at, "relax_p": self.relax_p})
+            self.guard_events.append({"detected_at": at, "restored_to": int(snap["iteration"]),
+                                      "relax_p": round(self.relax_p, 4), "continuity": r})
             return True
         stage2 = self.algo == "SIMPLEC" or self.relax_u * 0.7 >= self.GUARD_MIN_RELAX_U - 1e-12
         if bad and self._guard_snap is not None and stage2:
@@ -789,7 +790,9 @@ class Simplec:
                 self.relax_u *= 0.7
             self._amg_age = 10 ** 9
             self._guard_min = snap_r
-            self.guard_events.append({"detected_at": at, "algorithm": self.algo})
+            self.guard_events.append({"detected_at": at, "restored_to": int(snap["iteration"]), "algorithm": self.algo,
+                                      "relax_p": round(self.relax_p, 4), "relax_u": round(self.relax_u, 4),
+                                      "continuity": r})
             return True
         if bad and self.guard_exhausted_at is None:
             self.guard_exhausted_at = self.iteration

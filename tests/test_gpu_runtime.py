@@ -64,6 +64,37 @@ def test_snapshot_checkpoint_and_restart(tmp_path):
         assert rel_l2(f[k], ref) < 1e-8, k
 
 
+def test_sst_snapshot_holds_the_turbulence_fields_and_restarts_bit_for_bit(tmp_path):
+    """A turbulent checkpoint carries k, w and mut beside the flow fields (the keys of
+    ``state_dict()``), and a fresh solver restarted from the file steps exactly like the one that
+    wrote it."""
+    from sst_cases import channel_quads
+
+    mesh, case = channel_quads(C, 10 ** 6)
+    s = SimpleSolver(mesh, case, "cuda")
+    for _ in range(8):
+        s.step()
+    w = CheckpointWriter(str(tmp_path), keep=2)
+    w.submit("it8", s, {"iteration": s.iteration})
+    ref8 = s.state_dict()
+    w.close()
+    assert w.last_error is None
+    st, _ = load_checkpoint(str(tmp_path / "it8.npz"))
+    assert set(ref8) <= set(st) and {"k", "w", "mut"} <= set(st)
+    for k in ("U", "p", "F", "Fb", "k", "w", "mut"):
+        np.testing.assert_array_equal(st[k], ref8[k])
+    mesh2, case2 = channel_quads(C, 10 ** 6)
+    g = SimpleSolver(mesh2, case2, "cuda")
+    g.load_state(st)
+    s.load_state(st)  # both restart from the file: the lagged gradients are rebuilt alike
+    for _ in range(5):
+        s.step()
+        g.step()
+    a, b = s.state_dict(), g.state_dict()
+    for k in ("U", "p", "F", "Fb", "k", "w", "mut"):
+        np.testing.assert_array_equal(a[k], b[k])
+
+
 def test_run_batch_solves_each_case(tmp_path):
     # two closed cavities (a lid-speed override each) and the open square-cylinder channel,
     # all at nu = 0.1: one launch group
