@@ -75,6 +75,8 @@ _SIGS = {
     "ftmi_timer_read": ([P, P], I32),
     "ftmi_bicgstab_work_bytes": ([I64], I64),
     "ftmi_bicgstab_jacobi": ([I64, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
+    "ftmi_bicgstab_noalloc_max_rows": ([], I64),
+    "ftmi_bicgstab_jacobi_noalloc": ([I64, P, P, P, P, P, P, F64, I64, I32, P, P, P], I32),
     "ftmi_bicgstab_ldu_sliced": ([I64, P, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
     "ftmi_ldu_slice": ([I64, P, P, P, P], I32),
     "ftmi_bicgstab_ldu": ([I64, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),

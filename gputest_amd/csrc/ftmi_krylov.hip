@@ -1576,3 +1576,124 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free BiCGSTAB
+// ftmi_bicgstab_jacobi_noalloc: the same solve as ftmi_bicgstab_jacobi (same kernels, same
+// arithmetic, bit-identical x) with its workspace in a file-scope device array instead of a
+// caller buffer, and no pinned status block: the host reads the device state block with a
+// plain copy after every chunk of iterations. Nothing is allocated on any call. The array is
+// one per device, so calls that overlap on one device race on it.
+namespace {
+
+constexpr int64_t kNoAllocRows = int64_t(1) << 18;
+constexpr int64_t kNoAllocBytes = (11 * kNoAllocRows * 8 + 255) / 256 * 256 + 512 + 2 * kPartBytes;
+constexpr int kNoAllocChunk = 8;  // iterations enqueued between two reads of the state block
+constexpr int kNoAllocDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+struct ScratchTable {
+    void* ptr[kNoAllocDevices] = {};
+};
+static ScratchTable g_scratch;
+
+// The device address of `scratch` on the current device, looked up once per device (the host
+// name of the array is not a device address).
+static void* scratch_ptr() {
+    int dev = -1;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("bicgstab_noalloc: hipGetDevice", e);
+        return nullptr;
+    }
+    if (dev < 0 || dev >= kNoAllocDevices) {
+        set_error_msg("bicgstab_noalloc: device index out of range");
+        return nullptr;
+    }
+    if (!g_scratch.ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t r = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (r != hipSuccess || !p) {
+            set_error("bicgstab_noalloc: hipGetSymbolAddress(scratch)", r);
+            return nullptr;
+        }
+        g_scratch.ptr[dev] = p;
+    }
+    return g_scratch.ptr[dev];
+}
+
+static int read_state(State* H, const State* S, hipStream_t st) {
+    hipError_t e = hipMemcpyAsync(H, S, sizeof(State), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        set_error("bicgstab_noalloc: reading the solver state", e);
+        return -(int)e;
+    }
+    return 0;
+}
+
+template <class Op, typename T>
+int bicgstab_run_noalloc(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, int64_t maxiter, int rel,
+                         void* work, int64_t* iters, double* res, hipStream_t st) {
+    BiWork<T> W = carve_bi<T>(n, work);
+    const int g = grid_red(n);
+    k_state_reset<<<1, 64, 0, st>>>(W.S, W.counter);
+    bi_init<Op, T, false><<<g, kBlock, 0, st>>>(n, A, b, x, W.r, W.rh, W.p[0], W.v[0], W.part[0]);
+    fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel, kNoPost);
+    FTMI_LAUNCH_CHECK("bicgstab_noalloc init");
+    State H;
+    int64_t k = 1;
+    for (;;) {
+        if (int e = read_state(&H, W.S, st)) return e;
+        if (H.st[I_DONE] || k > maxiter) break;
+        const int64_t stop = k + kNoAllocChunk - 1 < maxiter ? k + kNoAllocChunk - 1 : maxiter;
+        for (; k <= stop; ++k) {
+            const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+            bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
+                                               W.part[0]);
+            bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
+                                               (int)k);
+            bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
+                                          tol, (int)k);
+            fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, kNoPost);
+        }
+        FTMI_LAUNCH_CHECK("bicgstab_noalloc iteration");
+    }
+    bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, W.S);
+    FTMI_LAUNCH_CHECK("bicgstab_noalloc final");
+    if (H.st[I_DONE]) {
+        *iters = H.st[I_ITERS];
+        *res = H.sc[S_RES];
+        return H.st[I_CONV];
+    }
+    *iters = maxiter;
+    *res = H.sc[S_RES];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ftmi_bicgstab_noalloc_max_rows(void) { return kNoAllocRows; }
+
+int ftmi_bicgstab_jacobi_noalloc(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                                 const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
+                                 int relative_to_initial, int64_t* iters, double* res, void* stream) {
+    if (n <= 0) {
+        *iters = 0;
+        *res = 0.0;
+        return 1;
+    }
+    if (n > kNoAllocRows) {
+        set_error_msg("ftmi_bicgstab_jacobi_noalloc: n exceeds ftmi_bicgstab_noalloc_max_rows()");
+        return -1;
+    }
+    void* work = scratch_ptr();
+    if (!work) return -1;
+    CsrOp<double> A{indptr, indices, data};
+    return bicgstab_run_noalloc<CsrOp<double>, double>(n, A, b, inv_diag, x, tol, maxiter, relative_to_initial, work,
+                                                       iters, res, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -83,6 +83,19 @@ int ftmi_bicgstab_jacobi(int64_t n, const int32_t* indptr, const int32_t* indice
                          const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
                          int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
 
+/* new, no counterpart in the reference: ftmi_bicgstab_jacobi for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * solve and the same x, bit for bit, without the work argument: the workspace is a file-scope
+ * device array of the library (about 23 MB per device, held until the process ends), and the
+ * host reads the solver state with a plain device-to-host copy every 8 iterations. No call
+ * allocates device or host memory. n may not exceed ftmi_bicgstab_noalloc_max_rows() (262144;
+ * larger n returns -1 with a message). All calls on one device share the array: calls that
+ * overlap on one device (two streams, two threads) race and must be serialised by the caller. */
+int64_t ftmi_bicgstab_noalloc_max_rows(void);
+int ftmi_bicgstab_jacobi_noalloc(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data,
+                                 const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
+                                 int relative_to_initial, int64_t* iters, double* res, void* stream);
+
 /* new: opt-in timing of the BiCGSTAB p-update + SpMV kernel (HIP events around each launch on
  * its stream); enable resets the totals, read waits for the recorded events */
 /* new: capture mode (1) for recording solves in a HIP graph: the Krylov solvers enqueue every
