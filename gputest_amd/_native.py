@@ -106,6 +106,8 @@ _SIGS = {
     "ftmi_diag_vstats": ([I64, I32, I64, P, P, P, P, P], I32),
     "ftmi_parse_hex": ([P, I64, P, I64], I64),
     "ftmi_parse_f64": ([P, I64, P, I64], I64),
+    "ftmi_fluent_index": ([P, I64, P, I64, P], I64),
+    "ftmi_fluent_walk": ([P, I64, I64, P], I64),
     "ftmi_amgs_ldu_to_csr": ([I64] + [P] * 8, I32),
     "ftmi_amgs_diag": ([I64, I64] + [P] * 6, I32),
     "ftmi_amgs_strength": ([I64, I64, P, P, P, P, F64, P, P], I32),

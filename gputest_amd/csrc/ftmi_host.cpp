@@ -185,3 +185,207 @@ int64_t ftmi_parse_f64(const char* text, int64_t len, double* out, int64_t cap) 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ Fluent section index
+// One pass over a Fluent mesh file that finds every top-level section and the byte spans a
+// decoder needs, so the per-section decoding can work on spans without re-scanning.
+//
+// Table row (kFluentRow int64 per section): index, header begin, header end, body begin,
+// body end, flags. The header is the text inside the first inner group of a grid section
+// ("(zone first last type ...)"); the body is the data group: its text for ASCII sections,
+// and for binary sections (index >= 2000) the bytes from after its '(' up to the
+// "End of Binary Section" marker (the decoder takes the length the header implies). For the
+// dimension section the header is its number; for zone-name sections the first inner group.
+// Sections that are neither (comments, case settings, unknown indices) are skipped: ASCII ones
+// as balanced groups whose quoted strings may hold parentheses, binary ones up to their marker.
+// Bytes outside any section (a stray line) are skipped up to the next newline.
+namespace {
+
+constexpr int kFluentRow = 6;
+constexpr int kHasBody = 1;
+
+struct FluentScan {
+    const char* b;
+    int64_t n, p = 0;
+    int64_t err = 0;  // first error position + 1 (0: none)
+    int code = 0;     // what went wrong
+
+    bool fail(int c) {
+        if (!err) {
+            err = p + 1;
+            code = c;
+        }
+        return false;
+    }
+    void blanks() {
+        while (p < n && is_blank(b[p])) ++p;
+    }
+    bool take(char c) {
+        blanks();
+        if (p >= n || b[p] != c) return fail(1);
+        ++p;
+        return true;
+    }
+    // a word: up to a blank or a parenthesis
+    void word(int64_t& w0, int64_t& w1) {
+        blanks();
+        w0 = p;
+        while (p < n && !is_blank(b[p]) && b[p] != '(' && b[p] != ')') ++p;
+        w1 = p;
+    }
+    // "( ... )" with no inner parentheses: the text span, cursor after ')'
+    bool flat_group(int64_t& g0, int64_t& g1) {
+        if (!take('(')) return false;
+        g0 = p;
+        while (p < n && b[p] != ')') {
+            if (b[p] == '(') return fail(2);
+            ++p;
+        }
+        if (p >= n) return fail(3);
+        g1 = p++;
+        return true;
+    }
+    // skip from just after an opening '(' to after its matching ')', strings honoured
+    bool balanced_rest() {
+        int depth = 1;
+        while (depth && p < n) {
+            const char c = b[p];
+            if (c == '"') {
+                int64_t q = p + 1;
+                while (q < n && (b[q] != '"' || b[q - 1] == '\\')) ++q;
+                if (q >= n) return fail(4);
+                p = q + 1;
+                continue;
+            }
+            depth += (c == '(') - (c == ')');
+            ++p;
+        }
+        return depth == 0 ? true : fail(3);
+    }
+    // find the binary end marker from p; marker position in m, cursor after "... <index>)"
+    bool binary_end(int64_t& m) {
+        static const char kMark[] = "End of Binary Section";
+        const int64_t L = (int64_t)sizeof(kMark) - 1;
+        for (int64_t q = p; q + L <= n; ++q) {
+            if (b[q] == 'E' && std::memcmp(b + q, kMark, (size_t)L) == 0) {
+                m = q;
+                p = q + L;
+                int64_t w0, w1;
+                word(w0, w1);
+                return take(')');
+            }
+        }
+        return fail(5);
+    }
+};
+
+bool is_grid_section(int64_t idx) {
+    switch (idx) {
+    case 10: case 2010: case 3010: case 12: case 2012: case 3012:
+    case 13: case 2013: case 3013: case 18: case 2018:
+        return true;
+    default:
+        return false;
+    }
+}
+
+}  // namespace
+
+extern "C" int64_t ftmi_fluent_index(const char* buf, int64_t len, int64_t* table, int64_t cap, int64_t* err_pos) {
+    FluentScan s{buf, len};
+    int64_t rows = 0;
+    auto emit = [&](int64_t idx, int64_t h0, int64_t h1, int64_t b0, int64_t b1, int64_t flags) {
+        if (rows < cap) {
+            int64_t* r = table + rows * kFluentRow;
+            r[0] = idx, r[1] = h0, r[2] = h1, r[3] = b0, r[4] = b1, r[5] = flags;
+        }
+        ++rows;
+    };
+    bool ok = true;
+    while (ok) {
+        s.blanks();
+        if (s.p >= s.n) break;
+        if (buf[s.p] != '(') {  // stray text between sections: skip the line
+            while (s.p < s.n && buf[s.p] != '\n') ++s.p;
+            continue;
+        }
+        ++s.p;
+        int64_t w0, w1;
+        s.word(w0, w1);
+        int64_t idx = 0;
+        bool numeric = w1 > w0 && w1 - w0 < 10;
+        for (int64_t q = w0; numeric && q < w1; ++q) {
+            if (buf[q] < '0' || buf[q] > '9') numeric = false;
+            else idx = idx * 10 + (buf[q] - '0');
+        }
+        if (!numeric) {
+            ok = s.balanced_rest();
+            continue;
+        }
+        if (idx == 2) {  // (2 dim)
+            int64_t d0, d1;
+            s.word(d0, d1);
+            ok = s.take(')');
+            if (ok) emit(idx, d0, d1, 0, 0, 0);
+        } else if (is_grid_section(idx)) {
+            int64_t h0, h1;
+            ok = s.flat_group(h0, h1);
+            if (!ok) break;
+            s.blanks();
+            if (s.p < s.n && buf[s.p] == ')') {  // a declaration without data
+                ++s.p;
+                emit(idx, h0, h1, 0, 0, 0);
+                continue;
+            }
+            if (idx < 2000) {
+                int64_t g0, g1;
+                ok = s.flat_group(g0, g1) && s.take(')');
+                if (ok) emit(idx, h0, h1, g0, g1, kHasBody);
+            } else {
+                ok = s.take('(');
+                const int64_t b0 = s.p;
+                int64_t m = 0;
+                ok = ok && s.binary_end(m);
+                if (ok) emit(idx, h0, h1, b0, m, kHasBody);
+            }
+        } else if (idx == 39 || idx == 45) {  // (45 (id type name)(...))
+            int64_t h0, h1;
+            ok = s.flat_group(h0, h1);
+            while (ok) {
+                s.blanks();
+                if (s.p < s.n && buf[s.p] == '(') {
+                    ++s.p;
+                    ok = s.balanced_rest();
+                } else {
+                    break;
+                }
+            }
+            ok = ok && s.take(')');
+            if (ok) emit(idx, h0, h1, 0, 0, 0);
+        } else if (idx >= 2000) {
+            int64_t m = 0;
+            ok = s.binary_end(m);
+        } else {
+            ok = s.balanced_rest();
+        }
+    }
+    if (!ok || s.err) {
+        *err_pos = s.err ? s.err - 1 : s.p;
+        return -(int64_t)(s.code ? s.code : 9);
+    }
+    *err_pos = -1;
+    return rows > cap ? -100 - rows : rows;
+}
+
+// Start of every record of a mixed (polygonal) face section: record i begins with its node
+// count k_i and holds k_i + 3 integers (count, nodes, c0, c1). Returns the integers used, or -1
+// when the records run past nvals.
+extern "C" int64_t ftmi_fluent_walk(const int64_t* vals, int64_t nvals, int64_t nrec, int64_t* starts) {
+    int64_t p = 0;
+    for (int64_t i = 0; i < nrec; ++i) {
+        if (p >= nvals || vals[p] < 0) return -1;
+        starts[i] = p;
+        p += vals[p] + 3;
+    }
+    return p <= nvals ? p : -1;
+}

@@ -224,6 +224,16 @@ int ftmi_diag_vstats(int64_t n, int ncomp, int64_t stride, const double* x, cons
  * tokens; returns the count, -1 - k if token k is malformed, -2 - cap if there are more than cap. */
 int64_t ftmi_parse_hex(const char* text, int64_t len, int64_t* out, int64_t cap);
 int64_t ftmi_parse_f64(const char* text, int64_t len, double* out, int64_t cap);
+/* new: the top-level section table of a Fluent mesh file (the section grammar of
+ * flowtrace/mesh/fluent_msh.py:196-350, read natively): per section 6 int64 (index, header
+ * begin, header end, body begin, body end, flags; bit 0 = has a data group). Returns the section
+ * count, -100 - count when cap is too small, or a negative error code (1 expected parenthesis,
+ * 2 nested group in grid data, 3 unterminated group, 4 unterminated string, 5 no end marker of a
+ * binary section) with the byte position in *err_pos. */
+int64_t ftmi_fluent_index(const char* buf, int64_t len, int64_t* table, int64_t cap, int64_t* err_pos);
+/* new: record starts of a mixed face section (each record: k, k nodes, c0, c1); returns the
+ * integers used or -1 when the records run past nvals */
+int64_t ftmi_fluent_walk(const int64_t* vals, int64_t nvals, int64_t nrec, int64_t* starts);
 
 /* new: SA-AMG set-up on the device (amg.py:60-117, everything but the aggregation) */
 /* CSR data of the pressure matrix from its face form (he_slot: CSR slot of each half-edge) */

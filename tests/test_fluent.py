@@ -151,6 +151,26 @@ def test_native_parsers_match_python():
     assert nat.lib().ftmi_parse_f64(b" \n ", 3, out.ctypes.data, 10) == 0
 
 
+def test_section_index_skips_strings_settings_and_binary_sections():
+    """The native section table: a comment whose string holds parentheses, case settings with
+    nested groups and a foreign binary section are skipped; grid sections report their header
+    and data spans; broken structure is reported with its position."""
+    from gputest_amd.fluent import _index
+
+    buf = (b'(0 "a ) tricky ( comment")\n(2 2)\nstray line\n(37 (settings (a (b c)) "x)"))\n'
+           b"(2099 (1 2)(\x01)\x00)(\nEnd of Binary Section 2099)\n"
+           b"(10 (0 1 3 0 2))\n(10 (1 1 3 1 2)(\n0 0\n1 0\n0 1\n))\n(45 (2 wall w)())\n")
+    t = _index(buf)
+    assert [int(r[0]) for r in t] == [2, 10, 10, 45]
+    assert buf[t[0][1]:t[0][2]] == b"2"
+    assert buf[t[2][1]:t[2][2]] == b"1 1 3 1 2" and buf[t[2][3]:t[2][4]].split() == [b"0", b"0", b"1", b"0", b"0", b"1"]
+    assert t[1][5] == 0 and t[2][5] == 1
+    for bad, what in ((b'(0 "never closed)\n', "string"), (b"(10 (1 1 3 1 2)(\n0 0\n", "not closed"),
+                      (b"(3010 (1 1 1 1 2)(\x00\x00)", "End of Binary"), (b"(13 (1 1 2 3 2)(1 (2) 3))", "nested")):
+        with pytest.raises(FluentMeshError, match=what):
+            read_fluent_mesh(bad)
+
+
 def test_malformed_files_raise(tmp_path):
     with pytest.raises(FluentMeshError):
         read_fluent_mesh(b"(2 2)\n(10 (1 1 2 1 2)(\n0 0\n1 zz\n))\n")
