@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftmi.so")
-ABI_VERSION = 102
+ABI_VERSION = 103
 
 P = ctypes.c_void_p
 I32, I64, F64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -42,7 +42,7 @@ class FtmiAssembleArgs(ctypes.Structure):
                 ("diag", P), ("diag_r", P), ("off", P), ("b", P * 3), ("dP", P), ("resid", P),
                 ("inv_r", P), ("x0", P * 3), ("out_fp32", I32),
                 ("diag32", P), ("diag_r32", P), ("off32", P), ("b32", P * 3), ("inv_r32", P), ("ex32", P),
-                ("gam_f", P), ("gam_b", P), ("extra", P * 3)]
+                ("gam_f", P), ("gam_b", P), ("extra", P * 3), ("cell_resid", P)]
 
 
 class FtmiSstCoef(ctypes.Structure):
@@ -93,7 +93,7 @@ _SIGS = {
     "ftmi_amg_set_coarse_f32": ([P, I64, P, P, P], I32),
     "ftmi_amg_vcycle_f32": ([P, P, P, P], I32),
     "ftmi_pcg_ldu_f32": ([I64, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
-    "ftmi_bicgstab_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
+    "ftmi_bicgstab_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, I32, I32, P, P, P, P], I32),
     "ftmi_pcg_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, P, I32, I32, P, P, P, P, P], I32),
     "ftmi_cast_f64_f32": ([I64, P, P, P], I32),
     "ftmi_gather_f64": ([I64, P, P, P, P], I32),
@@ -107,6 +107,7 @@ _SIGS = {
     "ftmi_parse_hex": ([P, I64, P, I64], I64),
     "ftmi_parse_f64": ([P, I64, P, I64], I64),
     "ftmi_fluent_index": ([P, I64, P, I64, P], I64),
+    "ftmi_segment_sums": ([I64, P, I64, I64, P, I32, P, P], I32),
     "ftmi_fluent_walk": ([P, I64, I64, P], I64),
     "ftmi_amgs_ldu_to_csr": ([I64] + [P] * 8, I32),
     "ftmi_amgs_diag": ([I64, I64] + [P] * 6, I32),

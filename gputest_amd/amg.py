@@ -403,7 +403,7 @@ class AmgFP32:
                                                nat.ptr(R.col), nat.ptr(rv), nat.ptr(dv), *[nat.ptr(t) for t in v]),
                       "amg_add_level_f32")
         nc = src.coarse_n
-        self._cinv = to_f32(src._cinv)
+        self._cinv = self._coarse_f32()
         self._cb = torch.zeros(nc, dtype=torch.float32, device=dev)
         self._cx = torch.zeros(nc, dtype=torch.float32, device=dev)
         nat.check(L.ftmi_amg_set_coarse_f32(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb),
@@ -418,7 +418,24 @@ class AmgFP32:
             to_f32(P.val, pv)
             to_f32(R.val, rv)
             to_f32(dinv, dv)
-        to_f32(self.src._cinv, self._cinv)
+        self._cinv.copy_(self._coarse_f32())
+
+    def _coarse_f32(self) -> torch.Tensor:
+        """The coarse inverse in FP32. A regular system takes the FP64 inverse rounded. A singular
+        one (the projected pressure system) takes the inverse on the range of the coarse matrix:
+        the FP64 inverse of the shifted matrix has an eigenvalue of order 1e10 / |diag| along the
+        null vector, harmless in FP64, but in FP32 it lifts the rounding error of a projected
+        residual (1e-7 |r|) to a correction far above |r|, and the FP32 PCG stalls."""
+        from .krylov import to_f32
+
+        src = self.src
+        if not src.singular:
+            return to_f32(src._cinv)
+        Ad = src.coarse.to_scipy().toarray()
+        lam, V = np.linalg.eigh(0.5 * (Ad + Ad.T))
+        keep = np.abs(lam) > 1e-8 * np.abs(lam).max()
+        K = (V[:, keep] / lam[keep]) @ V[:, keep].T
+        return torch.as_tensor(np.ascontiguousarray(K, np.float32), device=self.device)
 
     def __call__(self, b: torch.Tensor) -> torch.Tensor:
         b = b.contiguous()

@@ -7,9 +7,11 @@ launch over the cells and faces of all cases, and the linear operators are block
 ``BatchSolver`` runs one ``SimpleSolver`` on that mesh, replaying each outer iteration from a
 HIP graph (``SimpleSolver.use_graph``), with the streaming kernels' workgroup cap picked by
 ``autotune_grid_cap``. Each case keeps its own boundary conditions; Reynolds numbers differ by
-lid speed at a shared viscosity. The Krylov reductions and the residual normalisation run over
-the whole batch, so the iterates differ from separate solves, while each converged field is the
-case's own steady solution (the batch is checked per case against the oracle). Closed cases fix
+lid speed at a shared viscosity. The Krylov reductions run over the whole batch, so the iterates
+differ from separate solves, while each converged field is the case's own steady solution (the
+batch is checked per case against the oracle). Residuals are normalised and convergence is
+judged case by case (``SimpleSolver.set_case_partition``); the stability guard and the
+relaxation factors are shared by a launch group. Closed cases fix
 their pressure level at a reference cell each (``SimpleSolver.set_reference_cells``).
 """
 from __future__ import annotations
@@ -25,8 +27,8 @@ import torch
 
 from . import _native as nat
 from .case import BoundarySpec, UnstructuredCase
-from .mesh import UnstructuredMesh, Zone, cartesian_mesh_2d, sort_faces_by_zone
-from .simple import SimpleSolver
+from .mesh import UnstructuredMesh, Zone, cartesian_mesh_2d, non_orthogonality_deg, sort_faces_by_zone
+from .simple import SimpleSolver, auto_relax_p
 
 
 def concatenate_meshes(meshes: Sequence[UnstructuredMesh], prefixes: Sequence[str]) -> UnstructuredMesh:
@@ -126,6 +128,7 @@ class BatchSolver:
         self.solver.use_graph = use_graph
         # every cavity is closed: one pressure reference cell per case
         self.solver.set_reference_cells(self.cell_offsets[:-1])
+        self.solver.set_case_partition(self.cell_offsets)  # each cavity converges on its own
         self.setup_seconds = time.perf_counter() - t0
 
     @property
@@ -189,7 +192,12 @@ def plan_batch(jobs: Sequence[BatchJob], max_cells: int = 50_000_000) -> dict:
             if j.solver != "pressure-based":
                 raise ValueError(f"solver {j.solver!r} is not available (pressure-based only)")
             meshes[j.name] = read_any_mesh(j.mesh)[0]
-            rows.append({"name": j.name, "cells": meshes[j.name].n_cells, "key": _shared_key(j.case)})
+            key = _shared_key(j.case)
+            if j.case.get("relax_p") is None:
+                # automatic pressure relaxation depends on the mesh (simple.py:53-64): cases
+                # share a solver only when they would pick the same value
+                key += f"|relax_p={auto_relax_p(non_orthogonality_deg(meshes[j.name]))}"
+            rows.append({"name": j.name, "cells": meshes[j.name].n_cells, "key": key})
         except Exception as exc:  # reported per job by run_batch
             errors[j.name] = f"{type(exc).__name__}: {exc}"
     groups: List[List[str]] = []
@@ -250,6 +258,8 @@ def run_batch(jobs: Sequence[BatchJob], precision: str = "fp64", device="cuda", 
             solver.use_graph = use_graph
             if closed:
                 solver.set_reference_cells(offs[closed])
+            if len(group) > 1:
+                solver.set_case_partition(offs)
             t1 = time.perf_counter()
             while not solver.finished:
                 solver.step()
@@ -263,12 +273,19 @@ def run_batch(jobs: Sequence[BatchJob], precision: str = "fp64", device="cuda", 
                 pre = f"c{k}/"
                 strip = lambda d: {z[len(pre):]: v for z, v in d.items() if z.startswith(pre)}  # noqa: E731
                 a, b = int(offs[k]), int(offs[k + 1])
+                if len(group) > 1:  # the case's own convergence within the batch
+                    at = int(solver.case_converged_at[k])
+                    its, conv = (at if at >= 0 else solver.iteration), bool(solver.case_converged[k])
+                    resid = dict(solver.case_residuals[k])
+                else:
+                    its, conv, resid = solver.iteration, bool(solver.converged), solver.residuals()["scaled"]
                 res = {"name": name, "mesh": by_name[name].mesh, "error": None, "cells": b - a,
                        "device": f"{solver.device} (batched launch {gi}, {len(group)} cases)",
-                       "iterations": solver.iteration, "converged": bool(solver.converged),
-                       "solve_seconds": solve_s, "residuals": solver.residuals()["scaled"],
+                       "iterations": its, "converged": conv, "solve_seconds": solve_s, "residuals": resid,
                        "mass_balance": strip(mb), "wall_forces": strip(forces), "heat_flow": strip(heat),
-                       "stability_rollbacks": len(solver.guard_events)}
+                       # the guard and the relaxation are shared by the launch group
+                       "stability_rollbacks": len(solver.guard_events), "group_iterations": solver.iteration,
+                       "group_stability_rollbacks": len(solver.guard_events)}
                 job = by_name[name]
                 if job.out_dir:
                     os.makedirs(job.out_dir, exist_ok=True)

@@ -21,6 +21,7 @@
 
 #include "../../include/ftmi.h"
 #include "ftmi_common.h"
+#include "ftmi_ops.h"
 
 using namespace ftmi;
 
@@ -59,8 +60,27 @@ struct LaneRows {
     }
 };
 
-inline int rows_tpr(int64_t n, int64_t nnz) { return n > 0 && (double)nnz / (double)n > 96.0 ? 64 : 1; }
+// Lanes per row: the V-cycle's choice (entries per row <= 8: 1 lane, the plain column-order
+// sum; <= 24: 4; <= 96: 16; else 64), so that the long rows of the coarse Galerkin levels are
+// read by several lanes at once instead of one lane walking hundreds of entries.
+inline int rows_tpr(int64_t n, int64_t nnz) {
+    const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
+    if (avg <= 8.0) return 1;
+    if (avg <= 24.0) return 4;
+    if (avg <= 96.0) return 16;
+    return 64;
+}
 inline int rows_grid(int64_t n, int tpr) { return grid_for(n * tpr); }
+
+#define ROWS_TPR_LAUNCH(tpr, KERNEL, grid, st, ...)                        \
+    do {                                                                   \
+        switch (tpr) {                                                     \
+        case 64: KERNEL<64><<<(grid), kBlock, 0, (st)>>>(__VA_ARGS__); break; \
+        case 16: KERNEL<16><<<(grid), kBlock, 0, (st)>>>(__VA_ARGS__); break; \
+        case 4: KERNEL<4><<<(grid), kBlock, 0, (st)>>>(__VA_ARGS__); break;   \
+        default: KERNEL<1><<<(grid), kBlock, 0, (st)>>>(__VA_ARGS__); break;  \
+        }                                                                  \
+    } while (0)
 
 // the diagonal entry (the row's one entry with col == row, else 0)
 template <int TPR>
@@ -114,8 +134,8 @@ __global__ void __launch_bounds__(kBlock) k_tentative(int64_t n, const int32_t* 
 // max(||y_prev||, 1e-300), ||y_prev|| re-added from the previous partials by every
 // workgroup), then y = (D^-1 A) x with the products (d_i a_ij) x_j of a row-by-row sparse
 // product, and the partials of ||y||^2 and ||x||^2. The first iteration takes x as given.
-// Rows are handled by TPR lanes: 1 keeps the column-order sum, wider groups (dense coarse
-// levels only) sum in a fixed lane tree.
+// Rows are handled by TPR lanes (rows_tpr): 1 keeps the column-order sum, wider groups sum in a
+// fixed lane tree.
 template <int TPR>
 __global__ void __launch_bounds__(kBlock) k_pow_step(int64_t n, const int32_t* __restrict__ ptr,
                                                      const int32_t* __restrict__ col, const double* __restrict__ val,
@@ -138,9 +158,12 @@ __global__ void __launch_bounds__(kBlock) k_pow_step(int64_t n, const int32_t* _
     double acc[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)logical * (kBlock / TPR) + threadIdx.x / TPR; i < n; i += stride) {
         const double di = dinv[i];
-        double s = 0.0;
-        const int32_t e = ptr[i + 1];
-        for (int32_t j = ptr[i] + lane; j < e; j += TPR) s += (di * val[j]) * xv(col[j]);
+        // this lane's entries lane, lane + TPR, ... in order, four loads in flight at a time
+        const int32_t p0 = ptr[i] + lane, e = ptr[i + 1];
+        const int d = p0 < e ? (e - p0 + TPR - 1) / TPR : 0;
+        double s = gather_sum(
+            0.0, d, (int32_t)i, [&](int k) { return col[p0 + TPR * k]; },
+            [&](int k) { return di * val[p0 + TPR * k]; }, [&](int32_t j) { return xv(j); });
 #pragma unroll
         for (int off = TPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, TPR);
         if (lane == 0) {
@@ -677,10 +700,8 @@ int ftmi_amgs_ldu_to_csr(int64_t n, const int32_t* cptr, const int32_t* he_slot,
 int ftmi_amgs_diag(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val, double* diag,
                    double* dinv, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (rows_tpr(n, nnz) == 64)
-        k_diag<64><<<rows_grid(n, 64), kBlock, 0, s>>>(n, ptr, col, val, diag, dinv);
-    else
-        k_diag<1><<<rows_grid(n, 1), kBlock, 0, s>>>(n, ptr, col, val, diag, dinv);
+    const int tpr = rows_tpr(n, nnz);
+    ROWS_TPR_LAUNCH(tpr, k_diag, rows_grid(n, tpr), s, n, ptr, col, val, diag, dinv);
     FTMI_LAUNCH_CHECK("ftmi_amgs_diag");
     return 0;
 }
@@ -688,10 +709,8 @@ int ftmi_amgs_diag(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* co
 int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
                        const double* diag, double theta, uint8_t* mask, void* stream) {
     hipStream_t s = (hipStream_t)stream;
-    if (rows_tpr(n, nnz) == 64)
-        k_strength<64><<<rows_grid(n, 64), kBlock, 0, s>>>(n, ptr, col, val, diag, theta, mask);
-    else
-        k_strength<1><<<rows_grid(n, 1), kBlock, 0, s>>>(n, ptr, col, val, diag, theta, mask);
+    const int tpr = rows_tpr(n, nnz);
+    ROWS_TPR_LAUNCH(tpr, k_strength, rows_grid(n, tpr), s, n, ptr, col, val, diag, theta, mask);
     FTMI_LAUNCH_CHECK("ftmi_amgs_strength");
     return 0;
 }
@@ -709,20 +728,17 @@ int ftmi_amgs_power(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* c
                     const double* dinv, double* x, double* y, int iters, double* partials, double* out, void* stream) {
     if (n <= 0 || iters <= 0) return 0;
     hipStream_t s = (hipStream_t)stream;
-    const double avg = (double)nnz / (double)n;
-    const int tpr = avg > 96.0 ? 64 : 1;  // wide rows only on the dense coarse levels
+    // rows of up to 24 entries keep one lane, so the estimate is the column-order (reference)
+    // sum bit for bit where the hierarchy is parity-checked; longer rows take rows_tpr lanes
+    const int tpr = (double)nnz <= 24.0 * (double)n ? 1 : rows_tpr(n, nnz);
     int64_t g64 = (n * tpr + kBlock - 1) / kBlock;
     const int g = (int)(g64 < 1 ? 1 : (g64 > kRedGrid ? kRedGrid : g64));
     double* part[2] = {partials, partials + 2 * kRedGrid};
     double* yb[2] = {y, y + n};
     for (int it = 0; it < iters; ++it) {
         const int a = it & 1, c = a ^ 1;
-        if (tpr == 64)
-            k_pow_step<64><<<g, kBlock, 0, s>>>(n, ptr, col, val, dinv, yb[c], x, yb[a], it == 0, part[c], g,
-                                                part[a]);
-        else
-            k_pow_step<1><<<g, kBlock, 0, s>>>(n, ptr, col, val, dinv, yb[c], x, yb[a], it == 0, part[c], g,
-                                               part[a]);
+        ROWS_TPR_LAUNCH(tpr, k_pow_step, g, s, n, ptr, col, val, dinv, yb[c], x, yb[a], it == 0, part[c], g,
+                        part[a]);
     }
     k_pow_fin<<<1, kBlock, 0, s>>>(part[(iters - 1) & 1], g, &out[0]);
     FTMI_LAUNCH_CHECK("ftmi_amgs_power");

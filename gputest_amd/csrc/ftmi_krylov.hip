@@ -1474,7 +1474,8 @@ int ftmi_pcg_ldu_f32(int64_t n, const float* diag, const int32_t* ptr, const int
 
 int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, const int32_t* nbr, const float* off,
                             const float* b, const float* inv_diag, double* x, double tol, int64_t maxiter,
-                            int relative_to_initial, void* work, int64_t* iters, double* res, void* stream) {
+                            int relative_to_initial, int b_is_residual, void* work, int64_t* iters, double* res,
+                            void* stream) {
     if (n <= 0) {
         *iters = 0;
         *res = 0.0;
@@ -1485,9 +1486,13 @@ int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, cons
     float* rb = (float*)work;
     float* d = rb + n;
     void* inner = (char*)work + align256(2 * n * (int64_t)sizeof(float));
-    k_resid_f32<<<grid_for(n), kBlock, 0, st>>>(n, A, b, x, rb);
-    FTMI_LAUNCH_CHECK("ftmi_bicgstab_ldu_mixed residual");
-    const int conv = bicgstab_run<LduDiffOp<float>, float>(n, A, rb, inv_diag, d, tol, maxiter, relative_to_initial,
+    const float* r0 = b;  // b_is_residual: the caller formed b - A x (in higher precision)
+    if (!b_is_residual) {
+        k_resid_f32<<<grid_for(n), kBlock, 0, st>>>(n, A, b, x, rb);
+        FTMI_LAUNCH_CHECK("ftmi_bicgstab_ldu_mixed residual");
+        r0 = rb;
+    }
+    const int conv = bicgstab_run<LduDiffOp<float>, float>(n, A, r0, inv_diag, d, tol, maxiter, relative_to_initial,
                                                            inner, iters, res, st, true);
     if (conv < 0) return conv;
     k_add_f32<<<grid_for(n), kBlock, 0, st>>>(n, d, x);

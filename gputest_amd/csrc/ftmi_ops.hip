@@ -304,3 +304,34 @@ int ftmi_assemble(int64_t nnz, int64_t nc, int64_t nf, const int32_t* indptr, co
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ segmented sums
+// out[r * nseg + k] = sum over i in [off[k], off[k+1]) of x[r * stride + i] (|x| when absval):
+// one workgroup per (segment, row), strided per thread then a fixed tree, so the sums are the
+// same on every run. The per-case residual sums of a batch of cases on one concatenated mesh.
+namespace {
+__global__ void __launch_bounds__(kBlock) k_segment_sums(int64_t nseg, const int64_t* __restrict__ off,
+                                                         int64_t stride, const double* __restrict__ x, int absval,
+                                                         double* __restrict__ out) {
+    const int64_t k = blockIdx.x, r = blockIdx.y;
+    const double* row = x + r * stride;
+    double acc[1] = {0.0};
+    const int64_t lo = off[k], hi = off[k + 1];
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) acc[0] += absval ? fabs(row[i]) : row[i];
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) out[r * nseg + k] = acc[0];
+}
+}  // namespace
+
+extern "C" int ftmi_segment_sums(int64_t nseg, const int64_t* offsets, int64_t nrows, int64_t stride,
+                                 const double* x, int absval, double* out, void* stream) {
+    if (nseg <= 0 || nrows <= 0) return 0;
+    if (nseg > 2147483647 || nrows > 65535) {
+        set_error_msg("ftmi_segment_sums: too many segments or rows");
+        return -1;
+    }
+    k_segment_sums<<<dim3((unsigned)nseg, (unsigned)nrows), kBlock, 0, (hipStream_t)stream>>>(nseg, offsets, stride, x,
+                                                                                            absval, out);
+    FTMI_LAUNCH_CHECK("ftmi_segment_sums");
+    return 0;
+}

@@ -296,14 +296,19 @@ struct AsmArgs {
     double* dP;
     double* resid;  // resid[2*q] = sum |rhs - A phi|, resid[2*q+1] = sum |diag phi|
     double* x0[3];
+    double* cell_resid;  // nullptr, or the per-cell terms of resid: row j of (2 NF, nc)
 };
 
 // Arithmetic and outputs in R: double for the FP64 solver, float for the mixed-precision
 // transport systems (inputs stay FP64 and are rounded on load; dP stays FP64). In FP32 the
 // operator is also written in difference form: ex = diag_r + sum_k off_k, accumulated from
 // the face fluxes (each face adds +-F, the diffusion parts cancelling exactly) plus the
-// boundary and relaxation terms, and the residual sums use A phi = ex phi_P + sum off (phi_N
-// - phi_P); FP32 rounding then scales with the residual, not with |A| |phi|.
+// boundary and relaxation terms. The FP32 right-hand side b is then the residual of the
+// relaxed system at the current field, r = b_r - A_r phi = rhs - A phi, formed as
+// sum_b cb (phi_b - phi_P) - m phi_P - sum off (phi_N - phi_P) + sources (m: the net outflow,
+// in FP64), with the differences taken in FP64 and rounded once: FP32 rounding then scales
+// with the residual, not with |A| |phi|, whatever the level of phi (a temperature of 300 K
+// keeps its millikelvin differences). The FP32 solve finds the correction d of phi.
 template <typename R>
 struct AsmOut {
     R* diag;
@@ -329,10 +334,13 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
         R exs = R(0);                              // kDiff: sum of +-F over the faces
         // rt, rd: Cdiff products (owner +, neighbour -, interleaved in face order)
         R rb_[NF], rt[NF], rd[NF], ax[NF], pc[NF];
+        double pc64[NF];
+        double m64 = 0.0;  // kDiff: the cell's net outflow, sum of +-F and Fb (FP64)
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
             rb_[q] = rt[q] = rd[q] = ax[q] = R(0);
-            pc[q] = (R)A.phi[q][c];
+            pc64[q] = A.phi[q][c];
+            pc[q] = (R)pc64[q];
         }
         const int32_t e = m.cptr[c + 1];
         for (int32_t k = m.cptr[c]; k < e; ++k) {
@@ -347,7 +355,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             const R aNP = -(D + fmax0(F));
             const R a_here = isN ? aNP : aPN;
             O.off[k] = a_here;
-            if (kDiff) exs += isN ? -F : F;
+            if (kDiff) {
+                exs += isN ? -F : F;
+                m64 += isN ? -A.F[f] : A.F[f];
+            }
             if (isN) {
                 dg_n += D + fmax0(-F);
                 nb_n += -aNP;
@@ -360,8 +371,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
                 const double* g = A.grad[q];
-                const R po = (R)A.phi[q][o];
-                ax[q] += kDiff ? a_here * (po - pc[q]) : a_here * po;
+                const double po64 = A.phi[q][o];
+                const R po = (R)po64;
+                // kDiff: the neighbour difference is taken in FP64 and rounded once
+                ax[q] += kDiff ? a_here * (R)(po64 - pc64[q]) : a_here * po;
                 // non-orthogonal diffusion gamma (grad phi)_f . T
                 R gt = (wf * (R)g[P * DIM] + wn * (R)g[N * DIM]) * (R)m.Tv[f * DIM];
 #pragma unroll
@@ -401,8 +414,17 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             const R Db = (A.gam_b ? (R)A.gam_b[f] : gamma) * (R)m.alpha_b[f] * dir;
             dg_b += Db + fmax0(Fb);
             const R cb = Db + fmax0(-Fb);
+            if (kDiff) {
+                // difference form of the boundary source: cb (phi_b - phi_P), the rest of
+                // cb phi_b - (Db + max(Fb, 0)) phi_P being -Fb phi_P, which joins the net
+                // outflow term
+                m64 += A.Fb[f];
 #pragma unroll
-            for (int q = 0; q < NF; ++q) rb_[q] += cb * (R)A.phib[q][f];
+                for (int q = 0; q < NF; ++q) rb_[q] += cb * (R)(A.phib[q][f] - pc64[q]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < NF; ++q) rb_[q] += cb * (R)A.phib[q][f];
+            }
             if (A.dir_mode == 1)
                 qin += (bc.tkind[f] == T_FLUX ? R(1) : R(0)) * (R)bc.tval[f] * (R)m.Sbm[f] / (R)A.cp;
         }
@@ -428,10 +450,19 @@ __global__ void __launch_bounds__(kBlock) k_assemble(MeshV m, BcV bc, AsmArgs A,
             } else {
                 rhs = rhs + (A.extra[0] ? qin + (R)A.extra[0][c] : qin);  // energy source after the flux walls
             }
-            const R Aphi = (kDiff ? ex_u : diag) * pc[q] + ax[q];
-            acc[2 * q] += fabs(rhs - Aphi);
-            acc[2 * q + 1] += fabs(diag * pc[q]);
-            O.b[q][c] = rhs + crel * diag * pc[q];
+            // kDiff: rhs holds the boundary source in difference form, so the residual is
+            // rhs - m phi_P - sum off (phi_N - phi_P), every term small where the field is
+            // smooth whatever its level (no cancellation of a_P phi against b in FP32); the
+            // FP32 system's right-hand side is that residual, for the correction of phi
+            const R resid = kDiff ? (rhs - (R)m64 * pc[q]) - ax[q] : rhs - (diag * pc[q] + ax[q]);
+            const R r_abs = fabs(resid), d_abs = fabs(diag * pc[q]);
+            acc[2 * q] += r_abs;
+            acc[2 * q + 1] += d_abs;
+            if (A.cell_resid) {
+                A.cell_resid[(2 * q) * m.nc + c] = (double)r_abs;
+                A.cell_resid[(2 * q + 1) * m.nc + c] = (double)d_abs;
+            }
+            O.b[q][c] = kDiff ? resid : rhs + crel * diag * pc[q];
             if (A.x0[q]) A.x0[q][c] = A.phi[q][c];
         }
         if (A.simplec >= 0 && A.dP) {
@@ -762,6 +793,7 @@ int ftmi_transport_assemble(const ftmi_mesh* mesh, const ftmi_bc* bc, const ftmi
     A.cp = a->cp;
     A.dP = a->dP;
     A.resid = a->resid;
+    A.cell_resid = a->cell_resid;
     for (int q = 0; q < 3; ++q) A.x0[q] = q < nfld ? a->x0[q] : nullptr;
     hipStream_t s = (hipStream_t)stream;
     const int g = grid_part(m.nc);

@@ -233,6 +233,19 @@ def sort_faces_by_zone(mesh: UnstructuredMesh) -> UnstructuredMesh:
                             periodic=periodic)
 
 
+def non_orthogonality_deg(mesh: UnstructuredMesh) -> float:
+    """Largest angle between an interior face's area vector and the line joining its two cell
+    centres (``fvmesh.py:120-121``), in degrees; periodic pairs are left out."""
+    geo = compute_geometry(mesh)
+    inner = mesh.neighbour >= 0
+    if not inner.any():
+        return 0.0
+    S = geo.face_area_vec[inner]
+    d = geo.cell_centre[mesh.neighbour[inner]] - geo.cell_centre[mesh.owner[inner]]
+    c = np.einsum("fd,fd->f", S, d) / np.sqrt(np.einsum("fd,fd->f", S, S) * np.einsum("fd,fd->f", d, d))
+    return float(np.degrees(np.arccos(np.clip(c, -1.0, 1.0))).max())
+
+
 def _land_on(src: np.ndarray, dst: np.ndarray, reach: float):
     """For every point of ``src``, the nearest point of ``dst`` no farther than ``reach``, found by
     a sort-and-sweep along the coordinate in which ``dst`` spreads most: the candidates of a point

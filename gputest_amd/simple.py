@@ -124,6 +124,7 @@ class SimpleSolver:
             self.turb = DeviceSST(self)
         self.guard_enabled = case.relax_p is None
         self._capture = None
+        self._parts = None  # per-case convergence of a batch (set_case_partition)
         self._reset_guard()
         self.setup_seconds = time.perf_counter() - t0
 
@@ -333,6 +334,8 @@ class SimpleSolver:
         a.cp = float(c.cp)
         a.diag, a.diag_r, a.off, a.dP = nat.ptr(self.diag), nat.ptr(self.diag_r), nat.ptr(self.off), nat.ptr(self.dP)
         a.resid, a.inv_r = nat.ptr(self.scal[_S_MOM:]), nat.ptr(self.inv_r)
+        if self._parts is not None:
+            a.cell_resid = nat.ptr(self._parts["mom"])
         if self.mixed:
             self._fp32_outputs(a, dim)
         if self.turb is not None:  # mu + mu_t, wall functions, transpose stress (simple.py:546-548)
@@ -350,7 +353,7 @@ class SimpleSolver:
         for k in range(dim):
             if self.mixed:
                 infos.append(bicgstab_ldu_mixed(fv, self.ex32, self.off32, self.b32[k], self.inv_r32,
-                                                self.Ustar[k], 0.1, 30, True, ws=self.ws))
+                                                self.Ustar[k], 0.1, 30, True, ws=self.ws, b_is_residual=True))
             else:
                 infos.append(bicgstab_ldu(fv, self.diag_r, self.off, self.bmom[k], self.inv_r, self.Ustar[k], 0.1,
                                           30, True, ws=self.ws, sliced_off=self.soff))
@@ -414,6 +417,7 @@ class SimpleSolver:
         """Host part of an outer iteration: one read of the reduced scalars, the residual
         normalisation, the convergence test and the stability guard."""
         c, dim = self.case, self.dim
+        cs = self._case_sums() if self._parts is not None else None
         sc = self.scal.cpu().numpy()  # the one host read of the iteration
         self._accumulate_timers()
         res = {}
@@ -455,7 +459,10 @@ class SimpleSolver:
             self._graph = None  # relaxation and fields changed under the recorded graph
             self._last_res, self._last_raw = prev
             return 1.0
-        if self.iteration > 5 and max(res.values()) < c.tol:
+        if cs is not None:
+            self._judge_cases(cs)
+            self.converged = bool(self.case_converged.all())
+        elif self.iteration > 5 and max(res.values()) < c.tol:
             self.converged = True
         return 1.0
 
@@ -475,6 +482,8 @@ class SimpleSolver:
         a.relax, a.second_order, a.simplec, a.cp = float(c.relax_T), int(c.scheme == "second-order"), -1, float(c.cp)
         a.diag, a.diag_r, a.off = nat.ptr(self.diagT), nat.ptr(self.diagT_r), nat.ptr(self.offT)
         a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(self.inv_r)
+        if self._parts is not None:
+            a.cell_resid = nat.ptr(self._parts["en"])
         if self.mixed:
             self._fp32_outputs(a, 1)
         if self.turb is not None:  # k/cp + mu_t / Pr_t (simple.py:781-782)
@@ -485,7 +494,7 @@ class SimpleSolver:
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
         if self.mixed:
             info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
-                                      True, ws=self.ws)
+                                      True, ws=self.ws, b_is_residual=True)
         else:
             if self.soff is not None:
                 self.fv.slice_values(self.offT, self.soff)
@@ -512,6 +521,82 @@ class SimpleSolver:
         self._pin = (pin, torch.as_tensor(he, device=dev))
         self.singular_p = False
         self._graph = None
+
+    def set_case_partition(self, cell_offsets):
+        """Judge convergence case by case on a mesh that holds several independent cases side
+        by side (a batch): case k owns cells ``cell_offsets[k]:cell_offsets[k+1]`` and, in the
+        same order, contiguous runs of the interior and of the boundary faces. Each case then has
+        the reference's residual normalisation of its own (``simple.py:675-703``: momentum and
+        energy sums over its cells, continuity scaled by its largest value of the first five
+        iterations and floored at 1e-3 of its own face flux), ``case_residuals`` holds them, and
+        the solve is converged when every case is, ``case_converged_at`` recording the first
+        iteration at which each case passed. The stability guard and the relaxation stay shared
+        by the batch. The per-case sums come from per-cell residual terms the assembly writes
+        and one deterministic segmented sum per quantity."""
+        fv, dev = self.fv, self.device
+        off = np.asarray(cell_offsets, np.int64)
+        if off[0] != 0 or off[-1] != self.nc or np.any(np.diff(off) <= 0):
+            raise ValueError("cell_offsets must rise from 0 to the cell count")
+        ncase = off.size - 1
+        case_of = np.repeat(np.arange(ncase), np.diff(off))
+
+        def runs(owner):
+            k = case_of[owner]
+            if k.size and np.any(np.diff(k) < 0):
+                raise ValueError("the faces of the cases are not grouped case by case")
+            return np.searchsorted(k, np.arange(ncase + 1), side="left")
+
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.int64), device=dev)  # noqa: E731
+        z = lambda *sh: torch.zeros(*sh, dtype=torch.float64, device=dev)  # noqa: E731
+        self._parts = {"n": ncase, "cells": t(off), "faces": t(runs(fv.P_h[:fv.n_if])), "bfaces": t(runs(fv.Pb_h)),
+                       "mom": z(2 * self.dim, self.nc), "en": z(2, self.nc) if self.energy else None,
+                       "out": z(2 * self.dim + 5, ncase)}
+        self.case_res_max5 = np.zeros(ncase)
+        self.case_residuals = [dict() for _ in range(ncase)]
+        self.case_converged = np.zeros(ncase, bool)
+        self.case_converged_at = np.full(ncase, -1, np.int64)
+        self._graph = None  # recorded launches must write the per-cell terms
+
+    def _case_sums(self):
+        """Per-case reductions of the iteration just enqueued, as a (rows, cases) host array:
+        momentum |r| and |a_P u| per component, then sum |mass|, sum |F|, sum |Fb|, and the
+        energy pair."""
+        P, L, s = self._parts, nat.lib(), nat.stream_ptr()
+        n, out, d2 = P["n"], P["out"], 2 * self.dim
+        op = P["out"].data_ptr()
+        row = lambda r: op + 8 * r * n  # noqa: E731
+        nat.check(L.ftmi_segment_sums(n, nat.ptr(P["cells"]), d2, self.nc, nat.ptr(P["mom"]), 0, row(0), s), "case sums")
+        nat.check(L.ftmi_segment_sums(n, nat.ptr(P["cells"]), 1, self.nc, nat.ptr(self.mass), 1, row(d2), s),
+                  "case sums")
+        nat.check(L.ftmi_segment_sums(n, nat.ptr(P["faces"]), 1, self.fv.n_if, nat.ptr(self.F), 1, row(d2 + 1), s),
+                  "case sums")
+        nat.check(L.ftmi_segment_sums(n, nat.ptr(P["bfaces"]), 1, self.fv.n_bf, nat.ptr(self.Fb), 1, row(d2 + 2), s),
+                  "case sums")
+        if self.energy:
+            nat.check(L.ftmi_segment_sums(n, nat.ptr(P["cells"]), 2, self.nc, nat.ptr(P["en"]), 0, row(d2 + 3), s),
+                      "case sums")
+        return out.cpu().numpy()
+
+    def _judge_cases(self, cs):
+        """The reference's residual normalisation and convergence test, case by case."""
+        c, dim, names = self.case, self.dim, ("x-velocity", "y-velocity", "z-velocity")
+        cont = cs[2 * dim]
+        if self.iteration <= 5:
+            self.case_res_max5 = np.maximum(self.case_res_max5, np.maximum(cont, 1e-300))
+        flux = cs[2 * dim + 1] + cs[2 * dim + 2]
+        for k in range(self._parts["n"]):
+            den = [cs[2 * q + 1, k] for q in range(dim)]
+            floor = 1e-3 * max(max(den), 1e-300)
+            r = {names[q]: cs[2 * q, k] / max(den[q], floor) for q in range(dim)}
+            r["continuity"] = cont[k] / max(self.case_res_max5[k], 1e-3 * flux[k])
+            if self.energy:
+                rr, sd = cs[2 * dim + 3, k], cs[2 * dim + 4, k]
+                r["energy"] = rr / sd if sd > 0 else rr
+            self.case_residuals[k] = {q: float(v) for q, v in r.items()}
+            ok = self.iteration > 5 and max(r.values()) < c.tol
+            self.case_converged[k] = ok
+            if ok and self.case_converged_at[k] < 0:
+                self.case_converged_at[k] = self.iteration
 
     def _fp32_outputs(self, a, nfields):
         """Point a transport assembly at the FP32 system buffers (mixed precision)."""

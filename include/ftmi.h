@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 102
+#define FTMI_ABI_VERSION 103
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -143,14 +143,16 @@ int ftmi_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const
 int ftmi_pcg_ldu_f32(int64_t n, const float* diag, const int32_t* ptr, const int32_t* nbr, const float* off,
                      const float* b, float* x, double tol, int64_t maxiter, void* amg, int project, void* work,
                      int64_t* iters, double* res, void* stream);
-/* transport solve in correction form: rb = b - A x in FP32, Jacobi-BiCGSTAB in FP32 on
- * A d = rb from d = 0 (the same iterates and stopping rule as the solve from x, with the
+/* transport solve in correction form: rb = b - A x in FP32 (or, with b_is_residual, rb = b as
+ * given: the residual the FP32 transport assembly forms in difference form), Jacobi-BiCGSTAB in
+ * FP32 on A d = rb from d = 0 (the same iterates and stopping rule as the solve from x, with the
  * absolute floor 1e-14 ||rb||), then x += d in FP64. A is applied in difference form,
  * ex_i x_i + sum_k off_k (x_nbr - x_i), with ex the row sums (ftmi_assemble_args.ex32).
  * work: ftmi_bicgstab_work_bytes(n) */
 int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, const int32_t* nbr, const float* off,
                             const float* b, const float* inv_diag, double* x, double tol, int64_t maxiter,
-                            int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
+                            int relative_to_initial, int b_is_residual, void* work, int64_t* iters, double* res,
+                            void* stream);
 /* pressure solve by FP64 iterative refinement: r = b - A x in FP64 (projected when
  * `project`), stop when ||r|| / ||b|| < tol, else FP32 AMG-PCG on A32 d = r to the level
  * tol ||b||, its updates added straight into the FP64 x, at most max_refine times (FP32
@@ -323,8 +325,9 @@ typedef struct ftmi_assemble_args {
     double* inv_r;            /* 1 / diag_r (may be NULL) */
     double* x0[3];            /* copies of phi[q], the Krylov start vectors (may be NULL) */
     int out_fp32;             /* 1: FP32 arithmetic, and diag/diag_r/off/b/inv_r go to the FP32
-                                 arrays below (the mixed-precision transport systems); dP, resid
-                                 and x0 stay FP64 */
+                                 arrays below (the mixed-precision transport systems; b32 then
+                                 holds the residual b_r - A_r phi of the relaxed system at phi,
+                                 formed in difference form); dP, resid and x0 stay FP64 */
     float *diag32, *diag_r32, *off32, *b32[3], *inv_r32;
     float* ex32;              /* out_fp32: row sums diag_r + sum off of the relaxed operator,
                                  built from the face fluxes (the difference form) */
@@ -336,8 +339,17 @@ typedef struct ftmi_assemble_args {
                                  the turbulent transpose stress of simple.py:546-548); energy:
                                  extra[0] is added to the heat input after the flux walls
                                  (simple.py:795-796); NULL: none */
+    double* cell_resid;       /* per-cell residual terms, or NULL: (2 nfields, nc) doubles,
+                                 row 2q |rhs - A phi| and row 2q+1 |diag phi| of field q, whose
+                                 sums over a case are that case's residual sums (batches) */
 } ftmi_assemble_args;
 
+
+/* new: out[r nseg + k] = sum (or sum of |x| when absval) of x[r stride + i] over i in
+ * [offsets[k], offsets[k+1]), r < nrows (offsets: device, nseg + 1 entries, nondecreasing,
+ * offsets[nseg] <= stride); deterministic. The per-case sums of a batch on one mesh. */
+int ftmi_segment_sums(int64_t nseg, const int64_t* offsets, int64_t nrows, int64_t stride, const double* x,
+                      int absval, double* out, void* stream);
 
 /* ---------------------------------------------------------------- k-omega SST
  * (flowtrace/unstructured/turbulence.py:116-437; new entry points, the reference computes the

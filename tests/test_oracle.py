@@ -179,3 +179,25 @@ def test_oracle_operators_match_reference_cpp_kernels(tri):
     lib.ft_assemble(ctypes.c_int64(fv.nnz), ctypes.c_int64(fv.nc), ctypes.c_int64(fv.n_if), p(i32(fv.slot_diag)),
                     p(i32(fv.slot_PN)), p(i32(fv.slot_NP)), p(d), p(a1), p(a2), p(data))
     assert rel(data, fv.matrix(d, a1, a2).data) == 0.0
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="oracle/_ref not built (reference sources absent)")
+def test_oracle_on_the_reference_kernels_follows_the_plain_oracle():
+    """The CPU baseline's solver (the oracle with the reference's compiled C++/OpenMP kernels for
+    the gradients, face interpolation, assembly, incidence products and BiCGSTAB,
+    ``oracle/native_cpu.py``) follows the plain NumPy oracle on the heated duct: same inner
+    iteration counts, fields within the kernels' summation-order round-off."""
+    from gputest_amd.mesh import duct_mesh
+    from oracle import native_cpu as N
+    from tests.golden_cases import duct_case
+
+    m = duct_mesh(40, 20)
+    a = O.Simplec(m, duct_case(O, max_iter=10))
+    b = N.with_reference_kernels(O.Simplec(m, duct_case(O, max_iter=10)))
+    for _ in range(10):
+        a.step()
+        b.step()
+        assert b.last_linear["momentum_iters"] == a.last_linear["momentum_iters"]
+        assert b.last_linear["pressure_iters"] == a.last_linear["pressure_iters"]
+    assert rel(b.U, a.U) < 1e-10 and rel(b.p, a.p) < 1e-10
+    assert rel(b.T - 300.0, a.T - 300.0) < 1e-5  # the unconverged 60-step energy BiCGSTAB amplifies round-off
