@@ -6,14 +6,20 @@ Re = 500, energy on (inlet T = 300, wall heat flux 5000, Pr = 0.7), FP64 SIMPLEC
 SA-AMG-PCG (pressure tolerance 0.1) and Jacobi-BiCGSTAB. A step is one outer iteration; the
 AMG hierarchy is rebuilt every 10 iterations inside the timed region, as the solver does.
 
-``value`` is the FP64 solver's rate; ``mixed`` times the mixed-precision solver (FP32 transport
-assembly and BiCGSTAB corrections, FP32 AMG-PCG refined in FP64) over the same warm-up and
-timed window from the same initial state.
+``value`` is the FP64 solver's rate over the start-up window (``--warmup`` then ``--steps``
+iterations from rest, with the stability guard's rollbacks); ``steady`` times the same solver
+after iteration ``--steady-from``. ``mixed`` times the mixed-precision solver (FP32 transport
+assembly and BiCGSTAB corrections, FP32 AMG-PCG refined in FP64) over the same windows.
+``equal_error`` runs both solvers to convergence on the 250 x 125 duct and compares them with
+the reference's converged FP64 solution. ``batch`` times 256 cavities in one launch,
+``adjoint`` the drag gradient on a square cylinder.
 
 Prints one JSON line (see ``--help``). ``roofline`` is measured live for the dominant kernel,
 the BiCGSTAB p-update fused with the face-operator SpMV (its launches in three further outer
-iterations bracketed by HIP events on the solver's stream); ``cpu_baseline`` times the CPU
-FP64 restatement of the same algorithm on a bounded sample of the same workload.
+iterations bracketed by HIP events on the solver's stream); ``cpu_baseline`` times the
+reference's CPU path (the oracle's host code on the reference's compiled C++/OpenMP kernels)
+on a bounded sample of the same workload, and ``gpu_vs_oracle`` compares the GPU fields with
+the oracle's after the same iterations.
 """
 from __future__ import annotations
 
@@ -267,12 +273,20 @@ def sst_leg(nx, ny, steps, warmup):
             "residuals": s.residuals()["scaled"], "last_linear": s.last_linear, **info}
 
 
-def cpu_baseline(nx, ny, iters):
-    """The oracle (CPU FP64 NumPy/SciPy restatement) on the same workload: set-up excluded,
-    one warm-up iteration (builds the first AMG hierarchy), then ``iters`` timed iterations."""
-    import numpy as np  # noqa: F401
+def cpu_baseline(nx, ny, warmup, iters):
+    """The reference's CPU path on the same workload: the oracle's restatement of the NumPy host
+    code with the reference's own C++/OpenMP kernels (``oracle/native_cpu.py``, compiled from
+    ``flowtrace/native/fvm_kernels.cpp`` into ``oracle/_ref``) for the gradients, face
+    interpolation, assembly, incidence and matrix products and BiCGSTAB; AMG and PCG in
+    NumPy/SciPy as in the reference without pyamg. Timed the reference's way
+    (``hardware/planner.py:312-331``): set-up excluded, ``warmup`` iterations, then the median of
+    ``iters`` synchronised outer iterations, AMG rebuilt at its cadence. Falls back to the plain
+    NumPy oracle (one thread) when ``oracle/_ref`` is absent. Returns the baseline and the
+    oracle's fields after warmup + iters iterations from the initial state."""
+    import numpy as np
 
     import oracle.simplec_oracle as O
+    from oracle import native_cpu as N
 
     mesh, _, parabolic = _workload(nx, ny)
     mu = 1.0 / 500.0
@@ -284,14 +298,86 @@ def cpu_baseline(nx, ny, iters):
                   viscosity=mu, energy=True, conductivity=mu / 0.7, init_T=300.0, pressure_tol=0.1,
                   max_iter=10 ** 9, tol=0.0)
     s = O.Simplec(mesh, case)
-    s.step()
-    t0 = time.perf_counter()
-    for _ in range(iters):
+    kernels = N.available()
+    if kernels:
+        N.with_reference_kernels(s)
+    cores = N.threads() if kernels else 1
+    for _ in range(warmup):
         s.step()
-    dt = time.perf_counter() - t0
-    return {"value": round(iters / dt, 4), "unit": "it/s", "cores": 1, "kind": "port",
-            "sample": f"{iters} outer iterations of the same {mesh.n_cells}-cell duct after 1 warm-up iteration "
-                      f"(AMG rebuilt every 10 iterations; NumPy/SciPy, single-threaded)"}
+    ts = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        s.step()
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    how = (f"oracle host code + the reference's C++/OpenMP kernels (oracle/_ref) on {cores} threads"
+           if kernels else "NumPy/SciPy oracle, single-threaded")
+    out = {"value": round(1.0 / med, 4), "unit": "it/s", "cores": cores, "kind": "port",
+           "sample": f"median of {iters} synchronised outer iterations of the same {mesh.n_cells}-cell duct after "
+                     f"{warmup} warm-up iterations (AMG rebuilt every 10 iterations); {how}",
+           "mean_it_per_s": round(iters / float(np.sum(ts)), 4)}
+    return out, {"U": s.U.copy(), "p": s.p.copy(), "T": s.T.copy(), "iterations": s.iteration}
+
+
+def gpu_vs_oracle(nx, ny, fields) -> dict:
+    """Relative L2 of the GPU FP64 fields against the oracle's after the same outer iterations
+    from the same initial state (pressure mean-free; temperature as T - 300)."""
+    import numpy as np
+
+    from gputest_amd.simple import SimpleSolver
+
+    mesh, case, _ = _workload(nx, ny)
+    s = SimpleSolver(mesh, case, "cuda")
+    for _ in range(fields["iterations"]):
+        s.step()
+    U, p, T = s.U.cpu().numpy().T, s.p.cpu().numpy(), s.T.cpu().numpy()
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    return {"iterations": fields["iterations"], "U": float(f"{rel(U, fields['U']):.3g}"),
+            "p": float(f"{rel(p - p.mean(), fields['p'] - fields['p'].mean()):.3g}"),
+            "T": float(f"{rel(T - 300.0, fields['T'] - 300.0):.3g}")}
+
+
+def equal_error_leg(nx=250, ny=125):
+    """Equal L2 error: the FP64 and mixed solvers on the duct workload at nx x ny run to
+    convergence (FP64 to scaled residuals of 1e-8; mixed to its test at 1e-6, then on to the
+    FP64 solution's iteration count) and compared with the reference's own converged FP64
+    solution (``tests/golden/duct<nx>x<ny>_converged.npz``); also the rates of both."""
+    import numpy as np
+    import torch
+
+    from gputest_amd.simple import SimpleSolver
+
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"duct{nx}x{ny}_converged.npz"))
+    until = int(g["iterations"])
+    out = {"workload": f"heated duct {nx}x{ny}x2 jittered triangles", "reference_iterations": until}
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    for precision in ("fp64", "mixed"):
+        mesh, case, _ = _workload(nx, ny, precision)
+        case.tol, case.max_iter = (1e-8 if precision == "fp64" else 1e-6), 3 * until
+        s = SimpleSolver(mesh, case, "cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        while not s.finished:
+            s.step()
+        passed = s.iteration
+        if precision == "mixed" and s.converged and until > s.iteration:
+            s.case.tol, s.converged, s.case.max_iter = 0.0, False, until
+            while not s.finished:
+                s.step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        U, p, T = s.U.cpu().numpy().T, s.p.cpu().numpy(), s.T.cpu().numpy()
+        out[precision] = {"converged_at": passed, "iterations": s.iteration, "it_per_s": round(s.iteration / dt, 2),
+                          "rel_l2_vs_reference": {"U": float(f"{rel(U, g['U']):.3g}"), "p": float(f"{rel(p, g['p']):.3g}"),
+                                                  "T-T_in": float(f"{rel(T - 300.0, g['T'] - 300.0):.3g}")}}
+        print(f"[bench:equal-error] {precision}: {out[precision]}", file=sys.stderr, flush=True)
+    return out
 
 
 def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
@@ -337,6 +423,27 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
     return s, dt, info
 
 
+def steady_window(s, until, steps):
+    """Rate of ``steps`` outer iterations timed once the solver has run to iteration ``until``
+    (past the start-up transient and the stability guard's rollbacks)."""
+    import torch
+
+    while s.iteration < until:
+        s.step()
+    g0, r0 = len(s.guard_events), s.amg_rebuilds
+    pits = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.step()
+        pits.append(s.last_linear.get("pressure_iters"))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"from_iteration": until, "steps": steps, "it_per_s": round(steps / dt, 3),
+            "ms_per_step": round(dt / steps * 1e3, 3), "guard_events": len(s.guard_events) - g0,
+            "amg_rebuilds": s.amg_rebuilds - r0, "pressure_iters_total": int(sum(p or 0 for p in pits))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,13 +451,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nx", type=int, default=1000)
     ap.add_argument("--ny", type=int, default=500)
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=20)
+    ap.add_argument("--cpu-warmup", type=int, default=3)
+    ap.add_argument("--steady-from", type=int, default=80, help="iteration after which the steady window is timed")
+    ap.add_argument("--equal-error", type=int, default=1, help="run the converged equal-error comparison (1) or not")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision solver (1) or not (0)")
-    ap.add_argument("--batch", type=int, default=0, help="also time a batch of this many cavities (256: the "
-                    "log-uniform 16k-200k-cell batch)")
-    ap.add_argument("--adjoint", type=int, default=0, help="also time the square-cylinder adjoint with this many "
-                    "cells per body side (100: about 0.47M triangles)")
+    ap.add_argument("--batch", type=int, default=256, help="also time a batch of this many cavities (256: the "
+                    "log-uniform 16k-200k-cell batch; 0: skip)")
+    ap.add_argument("--adjoint", type=int, default=8, help="also time the square-cylinder adjoint with this many "
+                    "cells per body side (100: about 0.47M triangles; 0: skip)")
     ap.add_argument("--sst", type=int, default=0, help="also time k-omega SST on the duct mesh (1) or not (0)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -363,15 +473,23 @@ def main():
 
     mesh, case, _ = _workload(args.nx, args.ny)
     s, dt, info = run_timed(SimpleSolver, mesh, case, args.warmup, args.steps, "fp64")
+    pres = pressure_solve_roofline(s, info)
     mixed = None
     if args.mixed:
         _, mcase, _ = _workload(args.nx, args.ny, "mixed")
         sm, dtm, infom = run_timed(SimpleSolver, mesh, mcase, args.warmup, args.steps, "mixed")
         mixed = {"value": round(args.steps / dtm, 3), "ms_per_step": round(dtm / args.steps * 1e3, 3),
                  "speedup_vs_fp64": round(dt / dtm, 3), "dtype": "f32 solves + f64 refinement",
-                 "rel_l2_vs_fp64": window_difference(sm, s), **infom}
+                 "window_rel_l2_vs_fp64": window_difference(sm, s),
+                 "window_note": "fields of the two solvers after the same start-up window: two trajectories "
+                                "(rollbacks and AMG schedules differ), not the error of mixed precision; the "
+                                "equal-error comparison at convergence is `equal_error`", **infom}
+        mixed["steady"] = steady_window(sm, args.steady_from, args.steps)
         del sm
-    pres = pressure_solve_roofline(s, info)
+    steady = steady_window(s, args.steady_from, args.steps)
+    if mixed is not None:
+        mixed["steady"]["speedup_vs_fp64"] = round(mixed["steady"]["it_per_s"] / steady["it_per_s"], 3)
+    print(f"[bench] steady windows: fp64 {steady}, mixed {mixed and mixed['steady']}", file=sys.stderr, flush=True)
     roof = measure_roofline(s)
     roof["copy_ceiling"] = copy_ceiling()
     calibration_copy()
@@ -383,18 +501,24 @@ def main():
                                   f"FP64 SIMPLEC + energy, SA-AMG-PCG tol 0.1",
                       "cells": s.nc, "interior_faces": s.fv.n_if, "precision": "fp64",
                       **info},
-           "roofline": roof, "pressure_solve": pres}
+           "steady": steady, "roofline": roof, "pressure_solve": pres}
+    del s
     if mixed is not None:
         out["mixed"] = mixed
+    if args.equal_error:
+        out["equal_error"] = equal_error_leg()
     if args.batch:
-        out["batch"] = batch_leg(args.batch, steps=10, warmup=3)
+        out["batch"] = batch_leg(args.batch, steps=5, warmup=3)
     if args.adjoint:
-        out["adjoint"] = adjoint_leg(args.adjoint, tol=1e-9, max_iter=40000, gmres_tol=1e-8)
+        out["adjoint"] = adjoint_leg(args.adjoint, tol=1e-10, max_iter=40000, gmres_tol=1e-8)
     if args.sst:
         out["sst"] = sst_leg(4 * args.nx, args.ny // 2, args.steps, args.warmup)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(args.nx, args.ny, args.cpu_iters)
+        out["cpu_baseline"], fields = cpu_baseline(args.nx, args.ny, args.cpu_warmup, args.cpu_iters)
+        out["gpu_vs_oracle"] = gpu_vs_oracle(args.nx, args.ny, fields)
+        print(f"[bench] cpu baseline {out['cpu_baseline']}; gpu vs oracle {out['gpu_vs_oracle']}", file=sys.stderr,
+              flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -201,3 +201,27 @@ def test_oracle_on_the_reference_kernels_follows_the_plain_oracle():
         assert b.last_linear["pressure_iters"] == a.last_linear["pressure_iters"]
     assert rel(b.U, a.U) < 1e-10 and rel(b.p, a.p) < 1e-10
     assert rel(b.T - 300.0, a.T - 300.0) < 1e-5  # the unconverged 60-step energy BiCGSTAB amplifies round-off
+
+
+@pytest.mark.skipif(not os.path.exists(REF_LIB), reason="oracle/_ref not built (reference sources absent)")
+def test_reference_energy_iterate_is_not_reproducible_across_its_own_paths():
+    """The heated-duct temperature mid-run is not an iterate-level quantity even for the
+    reference: its 60-step energy BiCGSTAB never meets its 1e-3 tolerance there, and the
+    reference's own two CPU paths (NumPy, and its C++/OpenMP kernels with their own summation
+    order) part by several percent in T - 300 after five outer iterations of the 250 x 125 duct
+    while U and p agree to 1e-9. This bounds what any port can be held to on T mid-run; T is
+    pinned by the captured iteration and at convergence instead."""
+    from gputest_amd.mesh import duct_mesh
+    from oracle import native_cpu as N
+    from tests.golden_cases import duct_case
+
+    m = duct_mesh(250, 125)
+    a = O.Simplec(m, duct_case(O, max_iter=5))
+    b = N.with_reference_kernels(O.Simplec(m, duct_case(O, max_iter=5)))
+    for _ in range(5):
+        a.step()
+        b.step()
+    assert rel(b.U, a.U) < 1e-9 and rel(b.p, a.p) < 1e-9
+    dT = float(np.linalg.norm(b.T - a.T) / np.linalg.norm(a.T - 300.0))
+    print("T - 300 relative difference between the reference's two CPU paths", dT)
+    assert dT > 1e-2
