@@ -459,8 +459,8 @@ def main():
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision solver (1) or not (0)")
     ap.add_argument("--batch", type=int, default=256, help="also time a batch of this many cavities (256: the "
                     "log-uniform 16k-200k-cell batch; 0: skip)")
-    ap.add_argument("--adjoint", type=int, default=8, help="also time the square-cylinder adjoint with this many "
-                    "cells per body side (100: about 0.47M triangles; 0: skip)")
+    ap.add_argument("--adjoint", type=int, default=32, help="also time the square-cylinder adjoint with this many "
+                    "cells per body side (32: 47,532 triangles; 100: about 0.47M; 0: skip)")
     ap.add_argument("--sst", type=int, default=0, help="also time k-omega SST on the duct mesh (1) or not (0)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -9,12 +9,15 @@ J(w; beta) (the drag on a wall zone, ``simple.py:970-1001``)
     dJ/dbeta = dJ/dbeta|_w + lam^T dPhi/dbeta,   (I - dPhi/dw)^T lam = (dJ/dw)^T.
 
 ``ResidualMap`` restates Phi in torch (FP64, on the solver's device) from the same
-discretisation as the HIP kernels. Its two linear solves are autograd nodes whose backward
-is the transposed solve on the device: the momentum operator transposed by swapping each
-face's two coefficients (Jacobi-BiCGSTAB on the face form) and the symmetric pressure
-operator (SA-AMG-PCG). The adjoint system is solved by restarted GMRES on vector-Jacobian
-products of Phi evaluated once at w* (the forward iterations are not stored: the single
-step is recomputed in the backward pass). ``drag_gradient`` returns dJ/d(inlet profile
+discretisation as the HIP kernels. The vector-Jacobian products of Phi are hand-written
+(``adjoint_vjp.Linearisation``): the transpose of every stage of the outer iteration, applied in
+reverse order from the intermediates of one forward pass at w*, with the two transposed
+solves on the device: the momentum operator transposed by swapping each face's two
+coefficients (Jacobi-BiCGSTAB on the face form) and the symmetric pressure operator
+(SA-AMG-PCG). The adjoint system is solved by restarted GMRES on those products (the forward
+iterations are not stored: the single step is recomputed for the backward pass). The same
+products through torch autograd over ``ResidualMap.step`` (whose two solves are autograd nodes
+with the same transposed solves) remain as a cross-check (``vjp="autograd"``). ``drag_gradient`` returns dJ/d(inlet profile
 control values) and dJ/d(nu); ``AdjointDrag`` wraps forward solve and adjoint backward as a
 ``torch.autograd.Function``. With the energy equation the state also holds T and its lagged
 gradient, Phi also takes the energy step (``simple.py:778-813``, after the corrections, with
@@ -537,30 +540,60 @@ def gmres(apply: Callable[[torch.Tensor], torch.Tensor], b: torch.Tensor, tol: f
 
 
 def drag_gradient(solver, profile: InletProfile, zone: str = "cylinder", inlet: str = "inlet", tol: float = 1e-10,
-                  restart: int = 200, maxiter: int = 1000, rmap: Optional[ResidualMap] = None):
+                  restart: int = 200, maxiter: int = 1000, rmap: Optional[ResidualMap] = None, vjp: str = "hand"):
     """dJ/d(u_ctrl) and dJ/d(nu) of the drag J on ``zone`` at the converged state of
     ``solver`` (run with ``profile`` as its inlet and nu = viscosity / density).
+    ``vjp``: "hand" (the hand-written transposes of ``adjoint_vjp``) or "autograd" (torch
+    autograd through ``ResidualMap.step``, kept as a cross-check).
     Returns (grad_u (n_ctrl,), grad_nu, J, AdjointInfo)."""
-    with torch.enable_grad():  # also when called from an autograd backward pass
-        rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile)
-        grads, J, info = _gradient(rm, tol, restart, maxiter)
-        return grads[0], grads[1], J, info
+    rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile)
+    grads, J, info = _GRADIENT[vjp](rm, tol, restart, maxiter)
+    return grads[0], grads[1], J, info
 
 
 def heat_gradient(solver, profile: InletProfile, zone: str = "outlet", wall_T_zone: str = "cylinder",
                   inlet: str = "inlet", tol: float = 1e-10, restart: int = 200, maxiter: int = 1000,
-                  rmap: Optional[ResidualMap] = None):
+                  rmap: Optional[ResidualMap] = None, vjp: str = "hand"):
     """dJ/d(u_ctrl) and dJ/d(T_wall) of the heat flow J into the domain through ``zone`` at the
     converged state of ``solver`` (energy on; ``wall_T_zone`` a fixed-temperature wall).
     Returns (grad_u (n_ctrl,), grad_T_wall, J, AdjointInfo)."""
-    with torch.enable_grad():
-        rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile, objective="heat",
-                                                       wall_T_zone=wall_T_zone)
-        grads, J, info = _gradient(rm, tol, restart, maxiter)
-        return grads[0], grads[2], J, info
+    rm = rmap if rmap is not None else ResidualMap(solver, zone, inlet, profile, objective="heat",
+                                                   wall_T_zone=wall_T_zone)
+    grads, J, info = _GRADIENT[vjp](rm, tol, restart, maxiter)
+    return grads[0], grads[2], J, info
+
+
+def _gradient_hand(rm, tol, restart, maxiter):
+    """The adjoint gradient with the hand-written transposes: one linearisation at the fixed
+    point, J_w and J_beta by hand, GMRES on lam = J_w^T + Phi_w^T lam, then J_beta + Phi_beta^T lam."""
+    from .adjoint_vjp import Linearisation
+
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        w, beta = rm.state(), rm.params()
+        lin = Linearisation(rm, w, beta)
+        J, gJw, gJb = lin.objective_grad()
+        shapes = lin.w
+
+        def vjp_w(v):
+            rm.vjp_count += 1
+            return _flat(lin.vjp(_unflat(v, shapes))[0])
+
+        lam, its, rel = gmres(lambda v: v - vjp_w(v), _flat(gJw), tol=tol, restart=restart, maxiter=maxiter)
+        gb = lin.vjp(_unflat(lam, shapes))[1]
+        grads = [gJb[i] + gb[i] for i in range(len(beta))]
+    if rm.device.type == "cuda":
+        torch.cuda.synchronize(rm.device)
+    info = AdjointInfo(its, rel, time.perf_counter() - t0, rm.vjp_count)
+    return [g.detach() for g in grads], float(J), info
 
 
 def _gradient(rm, tol, restart, maxiter):
+    with torch.enable_grad():  # also when called from an autograd backward pass
+        return _gradient_autograd(rm, tol, restart, maxiter)
+
+
+def _gradient_autograd(rm, tol, restart, maxiter):
     t0 = time.perf_counter()
     w = rm.state()
     beta = rm.params()
@@ -585,6 +618,9 @@ def _gradient(rm, tol, restart, maxiter):
         torch.cuda.synchronize(rm.device)
     info = AdjointInfo(its, rel, time.perf_counter() - t0, rm.vjp_count)
     return [g.detach() if torch.is_tensor(g) else torch.as_tensor(g) for g in grads], float(J.detach()), info
+
+
+_GRADIENT = {"hand": _gradient_hand, "autograd": _gradient}
 
 
 class AdjointDrag(torch.autograd.Function):

@@ -1,16 +1,25 @@
-"""The discrete adjoint on the CPU: the torch restatement of one outer iteration has the
-oracle's converged state as its fixed point, and the adjoint gradient (SciPy solves) equals
-central finite differences (h = 1e-6) of the oracle's converged drag. No reference
-counterpart: the reference has no adjoint, so finite differences of the restated forward
-are the pin (the forward itself is pinned to the reference's golden vectors)."""
+"""The discrete adjoint on the CPU. No reference counterpart: the reference has no adjoint, so
+the pins are the oracle (itself pinned to the reference's golden vectors):
+
+* the torch restatement of one outer iteration has the oracle's converged state as its fixed
+  point;
+* the hand-written vector-Jacobian products equal torch autograd's through the restatement;
+* the adjoint gradient (SciPy solves in the products) equals the independent tangent reference
+  (``oracle/tangent_oracle.py``: central-difference Jacobian-vector products of the oracle's own
+  outer iteration, nothing of ``gputest_amd.adjoint``) along a random direction over all 65
+  parameters, at the first and last profile points and in nu, and central finite differences
+  (h = 1e-6) of the oracle's converged drag along the random direction and at one profile
+  point."""
 import numpy as np
 import pytest
 import torch
 
 import oracle.simplec_oracle as O
 from oracle.adjoint_oracle import CpuResidualMap, solver_view
+from oracle.tangent_oracle import TangentReference
 from gputest_amd.adjoint import drag_gradient, heat_gradient
-from tests.adjoint_cases import case, energy_case, small_mesh, u_ctrl0
+from gputest_amd.adjoint_vjp import Linearisation
+from tests.adjoint_cases import HALF, case, energy_case, small_mesh, u_ctrl0
 
 NU = 1.0 / 40.0
 
@@ -41,6 +50,67 @@ def test_outer_iteration_restatement_has_the_oracle_fixed_point(base):
         assert err < 1e-11, (name, err)
     drag = float(rm.objective(w, beta).detach())
     assert abs(drag - o.wall_forces()["cylinder"]["total"][0]) <= 1e-13 * abs(drag)
+
+
+def _direction():
+    d = np.random.default_rng(7).standard_normal(65)
+    return d / np.linalg.norm(d)
+
+
+@pytest.mark.parametrize("energy", [False, True])
+def test_hand_written_products_equal_autograd(base, heated, energy):
+    """(dPhi/dw)^T v and (dPhi/dbeta)^T v by the hand-written transposes against torch
+    autograd through the restated iteration, for random v; and dJ/dw, dJ/dbeta."""
+    mesh, o, prof = heated if energy else base
+    kw = dict(objective="heat", wall_T_zone="cylinder") if energy else {}
+    rm = CpuResidualMap(solver_view(o), "outlet" if energy else "cylinder", "inlet", prof, **kw)
+    w, beta = rm.state(), rm.params()
+    out = rm.step(w, beta)
+    rng = np.random.default_rng(1)
+    bars = [torch.as_tensor(rng.standard_normal(tuple(t.shape))) for t in out]
+    ref = torch.autograd.grad(out, w + beta, grad_outputs=bars, allow_unused=True)
+    lin = Linearisation(rm, w, beta)
+    hw, hb = lin.vjp(bars)
+    for a, b in zip(hw + hb, ref):
+        b = torch.zeros_like(a) if b is None else b
+        assert float((a - b).abs().max()) <= 1e-12 * max(float(b.abs().max()), 1e-300)
+    J = rm.objective(w, beta)
+    gref = torch.autograd.grad(J, w + beta, allow_unused=True)
+    J2, gw, gb = lin.objective_grad()
+    assert float(J2) == float(J)
+    for a, b in zip(gw + gb, gref):
+        b = torch.zeros_like(a) if b is None else b
+        assert float((a - b).abs().max()) <= 1e-13 * max(float(b.abs().max()), 1e-300)
+
+
+def test_cpu_adjoint_matches_the_independent_tangent_reference(base):
+    """The adjoint gradient against the tangent reference built from the oracle alone, along a
+    random direction over all 65 parameters, at profile points 0 and 63 (which no inlet face of
+    this mesh reaches: both derivatives are zero) and in nu; and central differences of the
+    oracle's converged drag along the same random direction."""
+    mesh, o, prof = base
+    rm = CpuResidualMap(solver_view(o), "cylinder", "inlet", prof)
+    gu, gnu, J, info = drag_gradient(None, prof, rmap=rm, tol=1e-11)
+    grad = np.r_[gu.numpy(), float(gnu)]
+    o2, _ = _solve(mesh, u_ctrl0(), NU)
+    tr = TangentReference(o2, u_ctrl0(), -HALF, HALF)
+    assert tr.fixed_point_residual() < 1e-12
+    d = _direction()
+    for name, v in (("random", d), ("k=0", np.eye(65)[0]), ("k=63", np.eye(65)[63]), ("nu", np.eye(65)[64])):
+        t, rec = tr.directional(v)
+        a = float(grad @ v)
+        print(name, "adjoint", a, "tangent", t, rec)
+        assert abs(a - t) <= 1e-5 * max(abs(t), 1e-3 * np.abs(grad).max()), (name, a, t)
+    h = 1e-6
+    u0 = u_ctrl0()
+
+    def drag(beta):
+        return _solve(mesh, beta[:64], beta[64])[0].wall_forces()["cylinder"]["total"][0]
+
+    b0 = np.r_[u0, NU]
+    fd = (drag(b0 + h * d) - drag(b0 - h * d)) / (2 * h)
+    print("random direction: adjoint", float(grad @ d), "central differences", fd)
+    assert abs(float(grad @ d) - fd) <= 1e-5 * abs(fd)
 
 
 def test_cpu_adjoint_matches_central_differences(base):
