@@ -204,7 +204,8 @@ def batch_leg(count, steps, warmup, seed=0):
     dt = time.perf_counter() - t0
     return {"cases": count, "cells": b.n_cells, "steps": steps, "it_per_s": round(steps / dt, 3),
             "ms_per_step": round(dt / steps * 1e3, 2), "cell_updates_per_s": round(b.n_cells * steps / dt, 1),
-            "grid_cap": cap, "graph_recordings": b.solver.graph_captures, "amg_rebuilds": b.solver.amg_rebuilds,
+            "grid_cap": cap, "graph_recordings": b.solver.graph_captures, "graph_redos": b.solver.graph_redos,
+            "amg_rebuilds": b.solver.amg_rebuilds,
             "setup_s": round(setup, 1), "precision": "mixed", "residuals": b.solver.residuals()["scaled"]}
 
 

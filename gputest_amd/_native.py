@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftmi.so")
-ABI_VERSION = 103
+ABI_VERSION = 104
 
 P = ctypes.c_void_p
 I32, I64, F64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -70,6 +70,8 @@ _SIGS = {
     "ftmi_assemble": ([I64, I64, I64] + [P] * 9, I32),
     "ftmi_assemble_plan_host": ([I64, I64, P, P, P, P, P], I32),
     "ftmi_capture_mode": ([I32], I32),
+    "ftmi_capture_plan": ([P, I32, P, I32], I32),
+    "ftmi_capture_slots_used": ([], I32),
     "ftmi_set_grid_cap": ([I32], I32),
     "ftmi_timer_enable": ([I32], I32),
     "ftmi_timer_read": ([P, P], I32),

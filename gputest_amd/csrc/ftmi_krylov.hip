@@ -39,7 +39,8 @@ namespace {
 // reads the previous one, so they live in different slots).
 // Rule for every kernel below: workgroup 0 may write a state slot only if no workgroup of the
 // same kernel reads that slot (workgroups that start later would see the new value).
-enum { S_RHO, S_ALPHA, S_OMEGA, S_RNORM, S_BNORM, S_BABS, S_RHONEW, S_RES, S_RZ, S_RZ1, S_ZMEAN, S_BMEAN, S_NS = 16 };
+enum { S_RHO, S_ALPHA, S_OMEGA, S_RNORM, S_BNORM, S_BABS, S_RHONEW, S_RES, S_RZ, S_RZ1, S_ZMEAN, S_BMEAN, S_TOLIN,
+       S_NS = 16 };
 enum { I_DONE, I_CONV, I_ITERS, I_CUR, I_FIXUP, I_NI = 8 };
 
 struct State {  // 192 bytes; `seq` stamps a host copy (see post_host)
@@ -48,6 +49,16 @@ struct State {  // 192 bytes; `seq` stamps a host copy (see post_host)
     unsigned seq;
     int pad[7];
 };
+
+// Report of a capped solve: ok if it finished (done) or ran its full maxiter (full).
+__global__ void k_report(const State* S, double* out, int full, int64_t cap) {
+    if (threadIdx.x != 0) return;
+    const int done = S->st[I_DONE];
+    out[0] = (done || full) ? 1.0 : 0.0;
+    out[1] = done ? (double)S->st[I_ITERS] : (double)cap;
+    out[2] = done ? (double)S->st[I_CONV] : 0.0;
+    out[3] = S->sc[S_RES];
+}
 
 __global__ void k_state_reset(State* s, unsigned* counter) {
     const int t = threadIdx.x;
@@ -172,11 +183,33 @@ struct KTimer {
 };
 static KTimer g_timer;
 
-// Capture mode: the solvers never read the device state back. Every iteration up to maxiter
+// Capture mode: the solvers never read the device state back. Every iteration up to its cap
 // is enqueued (kernels of a finished solve return at once on the device `done` flag) and the
 // final fix-ups read the state on the device, so a solve can be recorded in a HIP graph.
 // The iteration counts and residuals are then not reported (*iters = -1, *res = NaN).
 static bool g_capture = false;
+
+// Capture plan (ftmi_capture_plan): the solves recorded in capture mode take slots 0, 1, ...
+// in call order; slot s enqueues min(maxiter, caps[s]) iterations and, when it ends, writes
+// its report (4 doubles: ok, iterations, converged, residual) to report + 4 s on the device.
+// ok = 1 when the recorded iterations gave the eager solve's result (the solve finished
+// within its cap, or the cap was maxiter); the host redoes the step eagerly otherwise.
+struct CapturePlan {
+    std::vector<int64_t> caps;
+    double* report = nullptr;
+    int slots = 0;
+    int next = 0;
+};
+static CapturePlan g_plan;
+
+// The next slot's cap and report address (nullptr: no plan, or no room left).
+static double* capture_slot(int64_t maxiter, int64_t* cap) {
+    *cap = maxiter;
+    if (!g_capture || !g_plan.report) return nullptr;
+    const int s = g_plan.next++;
+    if (s < (int)g_plan.caps.size() && g_plan.caps[s] > 0 && g_plan.caps[s] < maxiter) *cap = g_plan.caps[s];
+    return s < g_plan.slots ? g_plan.report + 4 * s : nullptr;
+}
 
 // ------------------------------------------------------------------ BiCGSTAB kernels
 // One iteration is four launches: p-update fused into v = A phat (phat recomputed at the
@@ -489,7 +522,9 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
     fin_bi_init<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, rel, g_capture ? kNoPost : next_post(pn, 0));
     FTMI_LAUNCH_CHECK("bicgstab init");
     if (g_capture) {
-        for (int64_t k = 1; k <= maxiter; ++k) {
+        int64_t cap;
+        double* report = capture_slot(maxiter, &cap);
+        for (int64_t k = 1; k <= cap; ++k) {
             const int a = (int)((k - 1) & 1), c = (int)(k & 1);
             bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
                                                W.part[0]);
@@ -500,6 +535,7 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
             fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, kNoPost);
         }
         bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, W.S);
+        if (report) k_report<<<1, 64, 0, st>>>(W.S, report, cap >= maxiter ? 1 : 0, cap);
         FTMI_LAUNCH_CHECK("bicgstab (capture)");
         *iters = -1;
         *res = __builtin_nan("");
@@ -579,6 +615,16 @@ int ftmi_capture_mode(int on) {
     g_capture = on != 0;
     return was;
 }
+
+int ftmi_capture_plan(const int64_t* caps, int ncaps, double* report, int slots) {
+    g_plan.caps.assign(caps && ncaps > 0 ? caps : nullptr, caps && ncaps > 0 ? caps + ncaps : nullptr);
+    g_plan.report = report;
+    g_plan.slots = report ? slots : 0;
+    g_plan.next = 0;
+    return 0;
+}
+
+int ftmi_capture_slots_used(void) { return g_plan.next; }
 
 int ftmi_timer_read(double* total_ms, int64_t* launches) {
     g_timer.drain();
@@ -1097,8 +1143,9 @@ __global__ void __launch_bounds__(kBlock) pcg_init_b(int64_t n, const T* __restr
 }
 
 __global__ void __launch_bounds__(kBlock) fin_pcg_init(const double* partials, int nparts, State* S, double tol,
-                                                       Post post) {
+                                                       Post post, const double* tol_dev) {
     double tot[1];
+    if (tol_dev) tol = *tol_dev;
     if (finalize_sum<1>(partials, nparts, tot)) {
         const double rn = sqrt(tot[0]);
         S->sc[S_RNORM] = rn;
@@ -1212,8 +1259,9 @@ __global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const T* __restrict_
 }
 
 __global__ void __launch_bounds__(kBlock) fin_pcg_xr(const double* partials, int nparts, State* S, double tol, int k,
-                                                     Post post) {
+                                                     Post post, const double* tol_dev) {
     double tot[1];
+    if (tol_dev) tol = *tol_dev;
     if (!S->st[I_DONE] && finalize_sum<1>(partials, nparts, tot)) {
         const double rn = sqrt(tot[0]);
         S->sc[S_RNORM] = rn;
@@ -1260,7 +1308,8 @@ static PcgWork<T> carve_pcg(int64_t n, void* work) {
 // x in the iterate precision XT; zero_x = false adds the solve's result to the given x
 template <class Op, typename T, typename XT = T>
 int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg<T>* M, int project, void* work,
-            int64_t* iters, double* res, hipStream_t st, bool zero_x = true) {
+            int64_t* iters, double* res, hipStream_t st, bool zero_x = true, const double* tol_dev = nullptr,
+            int64_t cap = -1, double* report = nullptr) {
     Pinned* pn = pinned();
     if (!pn) {
         set_error_msg("pcg: pinned host memory unavailable");
@@ -1279,7 +1328,7 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
         pcg_init_a<false, T><<<g, kBlock, 0, st>>>(n, b, P0);
         pcg_init_b<false, T, XT><<<g, kBlock, 0, st>>>(n, b, W.r, x, zx, W.S, P0, g, P1);
     }
-    fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol, g_capture ? kNoPost : next_post(pn, 0));
+    fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol, g_capture ? kNoPost : next_post(pn, 0), tol_dev);
     // z = M r, with the partial sums of z in P2 when projecting; returns their count
     auto precond = [&]() -> int {
         if (M && M->nlev > 0) {
@@ -1308,17 +1357,19 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
         pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
     FTMI_LAUNCH_CHECK("pcg init");
     if (g_capture) {
-        for (int64_t k = 1; k <= maxiter; ++k) {
+        if (cap < 0) report = capture_slot(maxiter, &cap);
+        for (int64_t k = 1; k <= cap; ++k) {
             const int a = (int)((k - 1) & 1), c = (int)(k & 1);
             pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
             pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
-            fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, kNoPost);
+            fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, kNoPost, tol_dev);
             nz = precond();
             if (project)
                 pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
             else
                 pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         }
+        if (report) k_report<<<1, 64, 0, st>>>(W.S, report, cap >= maxiter ? 1 : 0, cap);
         FTMI_LAUNCH_CHECK("pcg (capture)");
         *iters = -1;
         *res = __builtin_nan("");
@@ -1329,7 +1380,7 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
         pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
         pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
-        fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, next_post(pn, last ^ 1));
+        fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, next_post(pn, last ^ 1), tol_dev);
         nz = precond();
         if (project)
             pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
@@ -1421,6 +1472,56 @@ __global__ void __launch_bounds__(kBlock) fin_mixed_norms(const double* p3, cons
     post_host(S, post);
 }
 
+// Capture mode, round 0 of the refinement: the inner tolerance tol ||b|| / ||r|| of the eager
+// round (x = 0, so ||r|| is the projected ||b||), kept on the device for the inner PCG.
+__global__ void k_mixed_tol_in(State* S0, double tol) {
+    if (threadIdx.x != 0) return;
+    const double babs = S0->sc[S_BABS], rn = S0->sc[S_RNORM];
+    const double bnorm = babs != 0.0 ? babs : 1.0;
+    S0->sc[S_TOLIN] = rn > 0.0 ? tol * bnorm / rn : tol;
+}
+
+// Report of a capped round 0 (S0: the norms before it, S1: after it, Sin: the inner PCG's
+// state): ok when the eager refinement would have ended after round 0 with this iterate.
+// Eager: rel0 < tol ends at once (the inner solve must then have been a no-op); else, with
+// refinement allowed, the inner PCG runs (it must have finished within its cap, or the cap was
+// maxiter), and the refinement ends after it on a breakdown at the start (x unchanged), on
+// rel1 < tol, or when no further round is allowed (max_refine 1, or maxiter used up).
+__global__ void k_report_mixed(const State* Sin, const State* S0, const State* S1, double tol, int ran, int full,
+                               int64_t cap, int max_refine, int64_t maxiter, double* out) {
+    if (threadIdx.x != 0) return;
+    const double babs = S0->sc[S_BABS];
+    const double bnorm = babs != 0.0 ? babs : 1.0;
+    const double rel0 = S0->sc[S_RNORM] / bnorm, rel1 = S1->sc[S_RNORM] / bnorm;
+    const int done = ran ? Sin->st[I_DONE] : 1;
+    const int64_t iters = !ran ? 0 : done ? (int64_t)Sin->st[I_ITERS] : cap;
+    const int cin = ran && done ? Sin->st[I_CONV] : 0;
+    double ok = 0.0, it = (double)iters, conv = 0.0, res = rel1;
+    if (rel0 < tol) {
+        ok = (!ran || (done && iters == 0)) ? 1.0 : 0.0;
+        it = 0.0;
+        conv = 1.0;
+        res = rel0;
+    } else if (!ran) {
+        ok = 1.0;
+        res = rel0;
+    } else if (done || full) {
+        if (iters == 0 && !cin) {
+            ok = 1.0;
+            res = rel0;
+        } else if (rel1 < tol) {
+            ok = 1.0;
+            conv = 1.0;
+        } else if (max_refine == 1 || iters >= maxiter) {
+            ok = 1.0;
+        }
+    }
+    out[0] = ok;
+    out[1] = it;
+    out[2] = conv;
+    out[3] = res;
+}
+
 // ex = (float)(d + sum_k off_k): the row sums of the FP64 operator, rounded once
 __global__ void __launch_bounds__(kBlock) k_rowsum_f32(int64_t n, LduOp<double> A, float* __restrict__ ex) {
     int64_t lo, hi;
@@ -1433,7 +1534,7 @@ __global__ void __launch_bounds__(kBlock) k_rowsum_f32(int64_t n, LduOp<double> 
 }
 
 static int64_t pcg_mixed_bytes(int64_t n) {
-    return align256(n * 8) + align256(2 * n * 4) + align256(sizeof(State)) + 2 * kPartBytes + pcg_bytes<float>(n);
+    return align256(n * 8) + align256(2 * n * 4) + align256(2 * sizeof(State)) + 2 * kPartBytes + pcg_bytes<float>(n);
 }
 
 }  // namespace
@@ -1519,8 +1620,8 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
     float* r32 = (float*)w;
     float* ex32 = r32 + n;
     w += align256(2 * n * 4);
-    State* S = (State*)w;
-    w += align256(sizeof(State));
+    State* S = (State*)w;  // S[0]: the norms of the current round; S[1]: capture mode, after round 0
+    w += align256(2 * sizeof(State));
     double* pa = (double*)w;
     double* pb = (double*)(w + kPartBytes);
     void* inner = w + 2 * kPartBytes;
@@ -1533,15 +1634,34 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
     int conv = 0;
     *rounds = 0;
     if (g_capture) {
-        // one correction, with the inner tolerance relative to the projected residual
+        // round 0 of the refinement, its inner PCG capped; the eager inner tolerance
+        // tol ||b|| / ||r|| is computed on the device; the report says whether the eager solve
+        // would have stopped after this round (else the host redoes the step eagerly)
+        int64_t cap;
+        double* report = capture_slot(maxiter, &cap);
         k_resid_f64<<<g, kBlock, 0, st>>>(n, A64, b, x, r64, pa);
         k_project_round<<<g, kBlock, 0, st>>>(n, r64, pa, g, project, r32, pb);
+        fin_mixed_norms<<<1, kBlock, 0, st>>>(pa, pb, g, S, kNoPost);
+        k_mixed_tol_in<<<1, 64, 0, st>>>(S, tol);
         FTMI_LAUNCH_CHECK("ftmi_pcg_ldu_mixed residual (capture)");
-        int64_t it = 0;
-        double r_in = 0.0;
-        const int c = pcg_run<LduDiffOp<float>, float, double>(n, A32, r32, x, tol, maxiter, M, project, inner, &it,
-                                                               &r_in, st, false);
-        if (c < 0) return c;
+        const int ran = max_refine >= 1 && maxiter > 0 ? 1 : 0;
+        if (ran) {
+            int64_t it = 0;
+            double r_in = 0.0;
+            const int c = pcg_run<LduDiffOp<float>, float, double>(n, A32, r32, x, tol, maxiter, M, project, inner,
+                                                                   &it, &r_in, st, false, &S->sc[S_TOLIN], cap,
+                                                                   nullptr);
+            if (c < 0) return c;
+        }
+        if (report) {
+            k_resid_f64<<<g, kBlock, 0, st>>>(n, A64, b, x, r64, pa);
+            k_project_round<<<g, kBlock, 0, st>>>(n, r64, pa, g, project, r32, pb);
+            fin_mixed_norms<<<1, kBlock, 0, st>>>(pa, pb, g, S + 1, kNoPost);
+            const State* Sin = carve_pcg<float>(n, inner).S;
+            k_report_mixed<<<1, 64, 0, st>>>(Sin, S, S + 1, tol, ran, cap >= maxiter ? 1 : 0, cap, max_refine,
+                                             maxiter, report);
+        }
+        FTMI_LAUNCH_CHECK("ftmi_pcg_ldu_mixed (capture)");
         *iters = -1;
         *res = __builtin_nan("");
         *rounds = 1;

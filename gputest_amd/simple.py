@@ -54,6 +54,10 @@ _S_SUMS = 7  # sum p_new, sum |F|, sum |Fb|
 _S_EN = 10  # energy: sum |b - A T|, sum |diag T|
 _S_TURB = 16  # SST: k then omega, each sum |r|, sum |diag phi|, max diag
 _NSCAL = 22
+# per-solve reports of a replayed graph (4 doubles each: ok, iterations, converged, residual),
+# after the reduced scalars so that one host read returns both: momentum components,
+# pressure, energy, SST k and omega
+_MAX_SOLVES = 8
 
 
 def auto_relax_p(non_orthogonality_deg: float) -> float:
@@ -84,6 +88,13 @@ class SimpleSolver:
         self.use_graph = False
         self._graph = None
         self.graph_captures = 0
+        # each recorded solve runs at most ceil(a * m) + b iterations, m the largest count of
+        # that solve since the previous recording (graph_cap = (a, b)); a replay in which a
+        # solve did not finish within its cap is redone eagerly (graph_redos counts them)
+        self.graph_cap = (1.25, 4)
+        self.graph_redos = 0
+        self._seen: list = []
+        self._saved = None
         self._pin = None
         self.max_refine = 4
         self._amg32 = None
@@ -203,7 +214,7 @@ class SimpleSolver:
             self.ex32 = f(nc)
             self.b32 = f(dim, nc)
             self.offp32 = f(2 * nf)
-        self.scal = z(_NSCAL)
+        self.scal = z(_NSCAL + 4 * _MAX_SOLVES)
         self.red_partials = z(2048 * 8)
         self.red_counter = torch.zeros(64, dtype=torch.int32, device=dev)
         r = nat.FtmiRed()
@@ -267,29 +278,106 @@ class SimpleSolver:
 
     def _graph_step(self) -> float:
         """One outer iteration replayed from a HIP graph of the device work. The graph is
-        recorded (in capture mode: Krylov solves without host reads) after each eager step that
-        rebuilds the AMG hierarchy, and replayed until the next rebuild; the host logic
-        (residual normalisation, convergence, guard) runs after every replay."""
+        recorded (in capture mode: Krylov solves without host reads, each capped near its recent
+        iteration counts) after each eager step that rebuilds the AMG hierarchy, and replayed
+        until the next rebuild. Each replay leaves a report per solve beside the reduced
+        scalars; when a solve did not finish within its cap, the iteration-start state that
+        the graph saved is put back and the iteration is redone eagerly (and the graph is
+        recorded again with the new counts). The host logic (residual normalisation,
+        convergence, guard) runs after every iteration."""
         due = self._amg is None or self._amg_age >= self.amg_rebuild or not self._has_lag
+        sc = None
         if due:
             self._graph = None
             out = self._launch(rebuild=True)
         else:
             if self._graph is None:
-                g = torch.cuda.CUDAGraph()
-                L = nat.lib()
-                prev = L.ftmi_capture_mode(1)
-                try:
-                    with torch.cuda.graph(g):
-                        self._graph_out = self._launch(rebuild=False)
-                finally:
-                    L.ftmi_capture_mode(prev)
-                self._graph = g
-                self.graph_captures += 1
+                self._record_graph()
             self._graph.replay()
-            out = self._graph_out
+            sc = self.scal.cpu().numpy()  # the one host read of the iteration
+            out = self._replay_infos(sc[_NSCAL:])
+            if out is None:
+                self._restore_saved()
+                self._graph = None
+                self.graph_redos += 1
+                sc = None
+                out = self._launch(rebuild=False)
+        self._note_counts(*out)
         self._amg_age += 1
-        return self._finish(*out)
+        return self._finish(*out, sc=sc)
+
+    def _iteration_state(self) -> Dict[str, torch.Tensor]:
+        """The device arrays an outer iteration reads before it writes them: the fields, the
+        lagged gradients of the boundary values, T and its gradient, the SST fields."""
+        st = {"U": self.U, "p": self.p, "F": self.F, "Fb": self.Fb, "gp": self.gp, "gU": self.gU}
+        if self.energy:
+            st.update(T=self.T, gT=self.gT)
+        if self.turb is not None:
+            st.update({"turb." + k: getattr(self.turb, k) for k in ("k", "w", "mut")})
+        return st
+
+    def _restore_saved(self):
+        for k, t in self._iteration_state().items():
+            t.copy_(self._saved[k])
+
+    def _record_graph(self):
+        L = nat.lib()
+        a, b = self.graph_cap
+        caps = [min(int(np.ceil(a * m)) + int(b), 1 << 30) for m in self._seen]
+        self._seen = []
+        arr = (ctypes.c_int64 * max(len(caps), 1))(*caps)
+        state = self._iteration_state()
+        if self._saved is None or set(self._saved) != set(state):
+            self._saved = {k: torch.empty_like(t) for k, t in state.items()}
+        nat.check(L.ftmi_capture_plan(arr, len(caps), nat.ptr(self.scal[_NSCAL:]), _MAX_SOLVES), "capture plan")
+        g = torch.cuda.CUDAGraph()
+        prev = L.ftmi_capture_mode(1)
+        try:
+            with torch.cuda.graph(g):
+                for k, t in state.items():  # the iteration-start state, for a redo
+                    self._saved[k].copy_(t)
+                self._graph_out = self._launch(rebuild=False)
+            used = L.ftmi_capture_slots_used()
+        finally:
+            L.ftmi_capture_mode(prev)
+            L.ftmi_capture_plan(None, 0, None, 0)
+        expect = len(self._solve_list(*self._graph_out))
+        if used != expect or used > _MAX_SOLVES:
+            raise RuntimeError(f"graph recording: {used} capped solves enqueued, {expect} expected")
+        self._graph = g
+        self.graph_captures += 1
+
+    def _solve_list(self, infos, pinfo, tinfo) -> list:
+        """The iteration's solves in the order they are enqueued (the capture plan's slots)."""
+        out = list(infos) + [pinfo] + ([tinfo] if tinfo is not None else [])
+        if self.turb is not None and self.turb.last_info:
+            out += [self.turb.last_info[k] for k in ("k", "omega")]
+        return out
+
+    def _note_counts(self, infos, pinfo, tinfo):
+        n = [int(i.iterations) for i in self._solve_list(infos, pinfo, tinfo)]
+        self._seen = n if len(self._seen) != len(n) else [max(x, y) for x, y in zip(self._seen, n)]
+
+    def _replay_infos(self, rep: np.ndarray):
+        """The Krylov infos of a replay from its reports, or None when a solve did not end
+        within its cap (or a report is missing)."""
+        infos, pinfo, tinfo = self._graph_out
+        nm = len(infos)
+        out = []
+        for s in range(len(self._solve_list(infos, pinfo, tinfo))):
+            ok, it, conv, res = (float(v) for v in rep[4 * s:4 * s + 4])
+            if ok != 1.0:
+                return None
+            it = int(it)
+            rounds = 0
+            if self.mixed and s == nm:  # FP32 corrections, as the eager refinement counts them
+                rounds = 0 if self.max_refine < 1 or (it == 0 and conv == 1.0) else 1
+            out.append(KrylovInfo(it, res, conv == 1.0, rounds))
+        minfo, pinfo = out[:nm], out[nm]
+        tinfo = out[nm + 1] if tinfo is not None else None
+        if self.turb is not None and self.turb.last_info:
+            self.turb.last_info = {"k": out[-2], "omega": out[-1]}
+        return minfo, pinfo, tinfo
 
     def _launch(self, rebuild: bool):
         """Enqueue the device work of one outer iteration; returns the Krylov infos."""
@@ -413,12 +501,13 @@ class SimpleSolver:
             self._mark("turbulence")
         return infos, pinfo, tinfo
 
-    def _finish(self, infos, pinfo, tinfo) -> float:
-        """Host part of an outer iteration: one read of the reduced scalars, the residual
-        normalisation, the convergence test and the stability guard."""
+    def _finish(self, infos, pinfo, tinfo, sc=None) -> float:
+        """Host part of an outer iteration: one read of the reduced scalars (unless the caller
+        read them), the residual normalisation, the convergence test and the stability guard."""
         c, dim = self.case, self.dim
         cs = self._case_sums() if self._parts is not None else None
-        sc = self.scal.cpu().numpy()  # the one host read of the iteration
+        if sc is None:
+            sc = self.scal.cpu().numpy()  # the one host read of the iteration
         self._accumulate_timers()
         res = {}
         num = [sc[_S_MOM + 2 * k] for k in range(dim)]

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 103
+#define FTMI_ABI_VERSION 104
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -99,10 +99,19 @@ int ftmi_bicgstab_jacobi_noalloc(int64_t n, const int32_t* indptr, const int32_t
 /* new: opt-in timing of the BiCGSTAB p-update + SpMV kernel (HIP events around each launch on
  * its stream); enable resets the totals, read waits for the recorded events */
 /* new: capture mode (1) for recording solves in a HIP graph: the Krylov solvers enqueue every
- * iteration up to maxiter without reading the device state back (a finished solve's kernels
- * return at once on the device), report *iters = -1 and *res = NaN, and the mixed pressure
- * solve applies exactly one FP32 correction. Returns the previous mode. */
+ * iteration up to maxiter (or the plan's cap) without reading the device state back (a
+ * finished solve's kernels return at once on the device), report *iters = -1 and *res = NaN,
+ * and the mixed pressure solve applies exactly one FP32 correction. Returns the previous mode. */
 int ftmi_capture_mode(int on);
+/* new: capture plan. The solves enqueued in capture mode take slots 0, 1, ... in call order
+ * (a mixed pressure solve takes one); slot s enqueues min(maxiter, caps[s]) iterations
+ * (caps[s] <= 0, or s >= ncaps: maxiter) and, when report != NULL and s < slots, writes
+ * four doubles to the device buffer report + 4 s when it ends: ok (1 when the enqueued
+ * iterations gave the eager solve's result), iterations, converged (0/1) and the residual
+ * the eager solve reports. report = NULL clears the plan. Resets the slot counter. */
+int ftmi_capture_plan(const int64_t* caps, int ncaps, double* report, int slots);
+/* new: slots taken since the last ftmi_capture_plan */
+int ftmi_capture_slots_used(void);
 /* new: workgroup cap of the streaming kernels (default 2048 = 8 per CU; 64..65536), the knob
  * the autotuner sets; returns the previous cap */
 int ftmi_set_grid_cap(int cap);

@@ -46,16 +46,41 @@ def test_fp64_batch_matches_each_case_at_convergence():
     _compare(b, 1e-6)
 
 
-def test_graph_replays_equal_eager_iterations():
-    eager = BatchSolver(SPECS, NU, precision="fp64", use_graph=False)
-    graph = BatchSolver(SPECS, NU, precision="fp64", use_graph=True)
-    for _ in range(25):  # spans hierarchy rebuilds and re-recordings
+def _replay_against_eager(precision, steps, cap=None):
+    eager = BatchSolver(SPECS, NU, precision=precision, use_graph=False)
+    graph = BatchSolver(SPECS, NU, precision=precision, use_graph=True)
+    if cap is not None:
+        graph.solver.graph_cap = cap
+    for it in range(steps):  # spans hierarchy rebuilds and re-recordings
         eager.step()
         graph.step()
-    assert graph.solver.graph_captures >= 2
+        # the solves' reports of a replay give the eager iteration counts and residuals
+        assert graph.solver.last_linear == eager.solver.last_linear, (it, graph.solver.last_linear,
+                                                                      eager.solver.last_linear)
+        assert graph.solver.residuals()["raw"] == eager.solver.residuals()["raw"], it
     for name in ("U", "p", "F", "Fb"):
         a, b = getattr(eager.solver, name), getattr(graph.solver, name)
         assert torch.equal(a, b), name
+    return graph.solver
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_equal_eager_iterations(precision):
+    """Capped recordings (each solve at most 1.25 m + 4 iterations, m its recent count); a
+    replay whose solve would have needed more, or (mixed) a second FP32 correction, is redone
+    eagerly, so every iteration equals the eager one bit for bit."""
+    g = _replay_against_eager(precision, 25)
+    print(precision, "recordings", g.graph_captures, "redone", g.graph_redos)
+    assert g.graph_captures >= 2
+    assert g.graph_redos <= 3
+
+
+def test_replays_past_their_caps_are_redone_eagerly():
+    """Caps of one iteration per solve: the pressure solve never finishes within its cap, so
+    every replay is rolled back to its saved start state and redone eagerly."""
+    g = _replay_against_eager("fp64", 14, cap=(0.0, 1))
+    print("recordings", g.graph_captures, "redone", g.graph_redos)
+    assert g.graph_redos >= 10 and g.graph_captures == g.graph_redos
 
 
 def test_mixed_precision_graph_batch_reaches_each_case():

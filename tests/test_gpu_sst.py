@@ -166,9 +166,10 @@ def test_sst_graph_replays_equal_eager_iterations():
     eager = SimpleSolver(mesh, case, DEV)
     graph = SimpleSolver(mesh, duct_tri(C, 25)[1], DEV)
     graph.use_graph = True
-    for _ in range(25):
+    for it in range(25):
         eager.step()
         graph.step()
+        assert graph.last_linear == eager.last_linear, (it, graph.last_linear, eager.last_linear)
     assert graph.graph_captures >= 2
     for name in ("U", "p", "F", "Fb", "T"):
         assert torch.equal(getattr(eager, name), getattr(graph, name)), name
