@@ -179,32 +179,40 @@ def calibration_copy(nbytes=256 << 20):
 
 
 def batch_leg(count, steps, warmup, seed=0):
-    """256 lid-driven cavities (cells log-uniform in [16k, 200k], quads/triangles 50/50,
-    Re uniform in [100, 1000], seeded) in one concatenated mesh, mixed precision, outer
-    iterations replayed from HIP graphs, streaming-kernel grid cap autotuned."""
+    """256 cases, lid-driven cavities and backward-facing steps 50/50 (cells log-uniform in
+    [16k, 200k], quads/triangles 50/50, Re uniform in [100, 1000], seeded) in one concatenated
+    mesh, mixed precision, outer iterations replayed from HIP graphs, the streaming kernels'
+    workgroup cap and the V-cycle's threads per row autotuned."""
     import torch
 
-    from gputest_amd.batch import BatchSolver, autotune_grid_cap, sample_cavities
+    from gputest_amd.batch import BatchSolver, autotune, sample_cases
 
-    specs = sample_cavities(count, seed=seed)
+    specs = sample_cases(count, seed=seed)
     print(f"[bench:batch] building {count} cases", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b = BatchSolver(specs, viscosity=0.01, precision="mixed", use_graph=True, max_iter=10 ** 9, tol=0.0)
     setup = time.perf_counter() - t0
     print(f"[bench:batch] {count} cases, {b.n_cells} cells, set-up {setup:.1f} s", file=sys.stderr, flush=True)
-    cap = autotune_grid_cap(b.solver, cache=None)
-    print(f"[bench:batch] grid cap {cap}", file=sys.stderr, flush=True)
+    tune = autotune(b.solver, cache=None)
+    cap = tune["cap"]
+    print(f"[bench:batch] tuned {tune}", file=sys.stderr, flush=True)
     for _ in range(warmup):
         b.step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    per = []
     for _ in range(steps):
-        b.step()
+        t1 = time.perf_counter()
+        b.step()  # the host reads the reduced scalars at the end of every step: a step is synchronous
+        per.append(time.perf_counter() - t1)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    return {"cases": count, "cells": b.n_cells, "steps": steps, "it_per_s": round(steps / dt, 3),
+    return {"cases": count, "median_ms_per_step": round(float(sorted(per)[len(per) // 2]) * 1e3, 2), "step_cases": sum(not sp.closed for sp in specs), "cells": b.n_cells, "steps": steps,
+            "it_per_s": round(steps / dt, 3),
             "ms_per_step": round(dt / steps * 1e3, 2), "cell_updates_per_s": round(b.n_cells * steps / dt, 1),
-            "grid_cap": cap, "graph_recordings": b.solver.graph_captures, "graph_redos": b.solver.graph_redos,
+            "grid_cap": cap, "tpr_bias": tune["tpr_bias"], "tuning_ms": tune["ms_per_iteration"],
+            "graph_recordings": b.solver.graph_captures, "graph_redos": b.solver.graph_redos,
+            "graph_record_s": round(b.solver.graph_record_seconds, 3),
             "amg_rebuilds": b.solver.amg_rebuilds,
             "setup_s": round(setup, 1), "precision": "mixed", "residuals": b.solver.residuals()["scaled"]}
 

@@ -72,6 +72,8 @@ _SIGS = {
     "ftmi_capture_mode": ([I32], I32),
     "ftmi_capture_plan": ([P, I32, P, I32], I32),
     "ftmi_capture_slots_used": ([], I32),
+    "ftmi_set_tpr_bias": ([I32], I32),
+    "ftmi_amg_retune": ([P], I32),
     "ftmi_set_grid_cap": ([I32], I32),
     "ftmi_timer_enable": ([I32], I32),
     "ftmi_timer_read": ([P, P], I32),

@@ -5,9 +5,10 @@ The cases' meshes are concatenated into one disconnected mesh (node, cell and fa
 appended with offsets, zones renamed per case), so every kernel of the outer iteration is one
 launch over the cells and faces of all cases, and the linear operators are block diagonal.
 ``BatchSolver`` runs one ``SimpleSolver`` on that mesh, replaying each outer iteration from a
-HIP graph (``SimpleSolver.use_graph``), with the streaming kernels' workgroup cap picked by
-``autotune_grid_cap``. Each case keeps its own boundary conditions; Reynolds numbers differ by
-lid speed at a shared viscosity. The Krylov reductions run over the whole batch, so the iterates
+HIP graph (``SimpleSolver.use_graph``), with the streaming kernels' workgroup cap and the
+V-cycle's threads per row picked by ``autotune``. The cases are lid-driven cavities (``CavitySpec``) and backward-facing
+steps (``StepSpec``), sampled by ``sample_cases``. Each case keeps its own boundary conditions;
+Reynolds numbers differ by lid or inlet speed at a shared viscosity. The Krylov reductions run over the whole batch, so the iterates
 differ from separate solves, while each converged field is the case's own steady solution (the
 batch is checked per case against the oracle). Residuals are normalised and convergence is
 judged case by case (``SimpleSolver.set_case_partition``); the stability guard and the
@@ -27,7 +28,7 @@ import torch
 
 from . import _native as nat
 from .case import BoundarySpec, UnstructuredCase
-from .mesh import UnstructuredMesh, Zone, cartesian_mesh_2d, non_orthogonality_deg, sort_faces_by_zone
+from .mesh import UnstructuredMesh, Zone, cartesian_mesh_2d, extract_cells, non_orthogonality_deg, sort_faces_by_zone
 from .simple import SimpleSolver, auto_relax_p
 
 
@@ -83,12 +84,67 @@ class CavitySpec:
     triangles: bool
     reynolds: float
 
+    closed = True  # no inflow or outflow: its pressure level is fixed at a reference cell
+
     def mesh(self) -> UnstructuredMesh:
         return cartesian_mesh_2d(self.n, self.n, triangulate=self.triangles)
 
     @property
     def cells(self) -> int:
         return self.n * self.n * (2 if self.triangles else 1)
+
+    def boundaries(self, prefix: str, viscosity: float) -> Dict[str, BoundarySpec]:
+        lid = self.reynolds * viscosity  # unit side
+        out = {prefix + "top": BoundarySpec("wall", velocity=[lid, 0.0])}
+        for side in ("left", "right", "bottom"):
+            out[prefix + side] = BoundarySpec("wall")
+        return out
+
+
+@dataclass
+class StepSpec:
+    """A backward-facing step of the batch (expansion ratio 2): the channel [0, length] x [0, 2]
+    without the step block [0, 1] x [0, 1], so that the flow enters over the step through the
+    inlet x = 0, y in [1, 2], and leaves through the outlet x = length; n cells per step height
+    h = 1 (quads, or two triangles per quad). Parabolic inlet of mean velocity U with
+    Re = U h / nu (the reference's step case, ``structured/case.py:215-231``: length 20 h,
+    step length h)."""
+    n: int
+    triangles: bool
+    reynolds: float
+    length: float = 20.0
+
+    closed = False
+
+    def mesh(self) -> UnstructuredMesh:
+        n, per = self.n, 2 if self.triangles else 1
+        nx, ny = int(round(self.length * n)), 2 * n
+        m = cartesian_mesh_2d(nx, ny, lx=self.length, ly=2.0, names=("inlet", "outlet", "bottom", "top"),
+                              types=("velocity-inlet", "pressure-outlet", "wall", "wall"), triangulate=self.triangles)
+        q = np.arange(m.n_cells) // per  # the quad of each cell: i ny + j
+        keep = ~((q // ny < n) & (q % ny < n))
+        return extract_cells(m, keep, name="step", type="wall")
+
+    @property
+    def cells(self) -> int:
+        return self.n * self.n * (2 * int(round(self.length)) - 1) * (2 if self.triangles else 1)
+
+    def boundaries(self, prefix: str, viscosity: float) -> Dict[str, BoundarySpec]:
+        U = self.reynolds * viscosity
+
+        def parabolic(x, U=U):  # mean U over y in [1, 2]
+            return np.stack([6.0 * U * (x[:, 1] - 1.0) * (2.0 - x[:, 1]), np.zeros(x.shape[0])], axis=1)
+
+        out = {prefix + "inlet": BoundarySpec("inlet", velocity=[U, 0.0], profile=parabolic),
+               prefix + "outlet": BoundarySpec("outlet")}
+        for side in ("bottom", "top", "step"):
+            out[prefix + side] = BoundarySpec("wall")
+        return out
+
+
+def _log_uniform_n(rng, cells, per_n2: float) -> int:
+    target = float(np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1]))))
+    return max(4, int(round(np.sqrt(target / per_n2))))
 
 
 def sample_cavities(count: int = 256, cells=(16_000, 200_000), reynolds=(100.0, 1000.0),
@@ -98,17 +154,32 @@ def sample_cavities(count: int = 256, cells=(16_000, 200_000), reynolds=(100.0, 
     rng = np.random.default_rng(seed)
     out = []
     for k in range(count):
-        target = float(np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1]))))
         tri = bool(k % 2)
-        n = max(4, int(round(np.sqrt(target / (2.0 if tri else 1.0)))))
+        n = _log_uniform_n(rng, cells, 2.0 if tri else 1.0)
         out.append(CavitySpec(n, tri, float(rng.uniform(*reynolds))))
     return out
 
 
-class BatchSolver:
-    """Independent lid-driven cavities solved together on one concatenated mesh."""
+def sample_cases(count: int = 256, cells=(16_000, 200_000), reynolds=(100.0, 1000.0), seed: int = 0,
+                 step_length: float = 20.0) -> list:
+    """Cavities and backward-facing steps, 50/50, each half 50/50 quads/triangles (case k:
+    triangles when k is odd, a step when k // 2 is odd); cell counts log-uniform in ``cells``,
+    Re uniform in ``reynolds`` (seeded)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        tri, step = bool(k % 2), bool((k // 2) % 2)
+        per = (2.0 if tri else 1.0) * ((2 * round(step_length) - 1) if step else 1.0)
+        n = _log_uniform_n(rng, cells, per)
+        re = float(rng.uniform(*reynolds))
+        out.append(StepSpec(n, tri, re, step_length) if step else CavitySpec(n, tri, re))
+    return out
 
-    def __init__(self, specs: Sequence[CavitySpec], viscosity: float = 0.01, precision: str = "mixed",
+
+class BatchSolver:
+    """Independent cavities and steps solved together on one concatenated mesh."""
+
+    def __init__(self, specs: Sequence, viscosity: float = 0.01, precision: str = "mixed",
                  device="cuda", use_graph: bool = True, meshes: Optional[Sequence[UnstructuredMesh]] = None,
                  **case_kw):
         t0 = time.perf_counter()
@@ -119,15 +190,14 @@ class BatchSolver:
         self.cell_offsets = np.concatenate([[0], np.cumsum([m.n_cells for m in meshes])])
         bnd = {}
         for k, s in enumerate(self.specs):
-            lid = s.reynolds * viscosity  # unit side
-            bnd[f"c{k}/top"] = BoundarySpec("wall", velocity=[lid, 0.0])
-            for side in ("left", "right", "bottom"):
-                bnd[f"c{k}/{side}"] = BoundarySpec("wall")
+            bnd.update(s.boundaries(prefixes[k], viscosity))
         self.case = UnstructuredCase(boundaries=bnd, viscosity=viscosity, precision=precision, **case_kw)
         self.solver = SimpleSolver(self.mesh, self.case, device)
         self.solver.use_graph = use_graph
-        # every cavity is closed: one pressure reference cell per case
-        self.solver.set_reference_cells(self.cell_offsets[:-1])
+        # each closed case (a cavity) fixes its pressure level at a reference cell
+        closed = [int(self.cell_offsets[k]) for k, s in enumerate(self.specs) if s.closed]
+        if closed:
+            self.solver.set_reference_cells(closed)
         self.solver.set_case_partition(self.cell_offsets)  # each cavity converges on its own
         self.setup_seconds = time.perf_counter() - t0
 
@@ -145,11 +215,13 @@ class BatchSolver:
             self.solver.step()
 
     def fields(self, k: int) -> Dict[str, np.ndarray]:
-        """Case k's velocity components and mean-free pressure (host arrays)."""
+        """Case k's velocity components and pressure (host arrays; mean-free for a closed
+        case, whose pressure level is arbitrary)."""
         a, b = int(self.cell_offsets[k]), int(self.cell_offsets[k + 1])
         f = self.solver.fields()
         out = {name: v[a:b].copy() for name, v in f.items() if v.shape[0] == self.n_cells}
-        out["pressure"] = out["pressure"] - out["pressure"].mean()
+        if self.specs[k].closed:
+            out["pressure"] = out["pressure"] - out["pressure"].mean()
         return out
 
 
@@ -307,59 +379,94 @@ def run_batch(jobs: Sequence[BatchJob], precision: str = "fp64", device="cuda", 
 _TUNE_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_build", "autotune.json")
 
 
-def autotune_grid_cap(solver: SimpleSolver, candidates=(1024, 2048, 4096, 8192), replays: int = 3,
-                      cache: Optional[str] = _TUNE_FILE) -> int:
-    """Workgroup cap of the streaming kernels that gives the fastest outer iteration of
-    ``solver`` (``replays`` graph-replayed iterations timed per candidate, each candidate in a
-    graph of its own). The choice is cached per (cells, interior faces, precision, device
-    name). The solver advances by the iterations it times."""
+def _retune_hierarchies(solver: SimpleSolver):
+    """Re-pick the threads per row of the solver's live AMG hierarchies with the current bias."""
+    L = nat.lib()
+    for amg in (solver._amg, getattr(solver, "_amg32", None)):
+        h = getattr(amg, "handle", None) if amg is not None else None
+        if h:
+            nat.check(L.ftmi_amg_retune(h), "amg_retune")
+
+
+def autotune(solver: SimpleSolver, caps=(1024, 2048, 4096), tpr_biases=(-1, 0, 1), replays: int = 3,
+             cache: Optional[str] = _TUNE_FILE) -> dict:
+    """Launch shapes that give the fastest outer iteration of ``solver``: the workgroup cap of
+    the streaming kernels (``caps``) and the shift of the V-cycle's threads per row on every
+    level (``tpr_biases``: steps of 1 / 4 / 16 / 64 lanes per row from the row-length rule).
+    Each candidate is recorded in a graph of its own and ``replays`` replayed iterations are
+    timed. The choice is cached per (cells, interior faces, precision, device name) and
+    applied; returns {"cap", "tpr_bias", "ms_per_iteration"}. The solver advances by the
+    iterations it times."""
     key = f"{solver.nc}:{solver.fv.n_if}:{solver.case.precision}:{torch.cuda.get_device_name(solver.device)}"
     L = nat.lib()
     if cache and os.path.exists(cache):
         with open(cache) as fh:
             known = json.load(fh)
-        if key in known:
-            L.ftmi_set_grid_cap(int(known[key]["cap"]))
-            return int(known[key]["cap"])
+        hit = known.get(key)
+        if hit and "tpr_bias" in hit and int(hit["cap"]) in caps and int(hit["tpr_bias"]) in tpr_biases:
+            L.ftmi_set_grid_cap(int(hit["cap"]))
+            L.ftmi_set_tpr_bias(int(hit["tpr_bias"]))
+            _retune_hierarchies(solver)
+            solver._graph = None
+            return hit
     if not solver.use_graph:
         raise ValueError("autotuning times graph replays: set solver.use_graph")
     times = {}
-    for cap in candidates:
-        L.ftmi_set_grid_cap(int(cap))
-        solver._graph = None  # launch shapes are recorded in the graph
-        # step (rebuilding the hierarchy if due) until a graph is recorded with room for the timed replays
-        for _ in range(2 * solver.amg_rebuild + 2):
-            if solver.finished:
+    for bias in tpr_biases:
+        for cap in caps:
+            L.ftmi_set_grid_cap(int(cap))
+            L.ftmi_set_tpr_bias(int(bias))
+            _retune_hierarchies(solver)
+            solver._graph = None  # launch shapes are recorded in the graph
+            # step (rebuilding the hierarchy if due) until a graph is recorded with room for the timed replays
+            for _ in range(2 * solver.amg_rebuild + 2):
+                if solver.finished:
+                    break
+                room = solver._amg is not None and solver._has_lag and \
+                    solver._amg_age + replays + 1 < solver.amg_rebuild
+                solver.step()
+                if room and solver._graph is not None:
+                    break
+            if solver.finished or solver._graph is None:
                 break
-            room = solver._amg is not None and solver._has_lag and \
-                solver._amg_age + replays + 1 < solver.amg_rebuild
-            solver.step()
-            if room and solver._graph is not None:
-                break
-        if solver.finished or solver._graph is None:
-            break
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(replays):
-            solver.step()
-        torch.cuda.synchronize()
-        times[int(cap)] = (time.perf_counter() - t) / replays * 1e3
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(replays):
+                solver.step()
+            torch.cuda.synchronize()
+            times[(int(cap), int(bias))] = (time.perf_counter() - t) / replays * 1e3
     if not times:
-        return int(L.ftmi_set_grid_cap(int(candidates[0])))
-    best = min(times, key=times.get)
-    L.ftmi_set_grid_cap(best)
+        L.ftmi_set_grid_cap(int(caps[0]))
+        L.ftmi_set_tpr_bias(0)
+        _retune_hierarchies(solver)
+        solver._graph = None
+        return {"cap": int(caps[0]), "tpr_bias": 0, "ms_per_iteration": {}}
+    cap, bias = min(times, key=times.get)
+    L.ftmi_set_grid_cap(cap)
+    L.ftmi_set_tpr_bias(bias)
+    _retune_hierarchies(solver)
     solver._graph = None
+    out = {"cap": cap, "tpr_bias": bias, "ms_per_iteration": {f"{c}/{b}": round(v, 3) for (c, b), v in times.items()}}
     if cache:
         os.makedirs(os.path.dirname(cache), exist_ok=True)
         known = {}
         if os.path.exists(cache):
             with open(cache) as fh:
                 known = json.load(fh)
-        known[key] = {"cap": best, "ms_per_iteration": times}
+        known[key] = out
         with open(cache, "w") as fh:
             json.dump(known, fh, indent=1)
-    return best
+    return out
 
 
-__all__ = ["concatenate_meshes", "CavitySpec", "sample_cavities", "BatchSolver", "autotune_grid_cap",
-           "BatchJob", "plan_batch", "run_batch"]
+def autotune_grid_cap(solver: SimpleSolver, candidates=(1024, 2048, 4096, 8192), replays: int = 3,
+                      cache: Optional[str] = _TUNE_FILE) -> int:
+    """``autotune`` over the workgroup cap alone (threads per row at the current bias)."""
+    L = nat.lib()
+    bias = int(L.ftmi_set_tpr_bias(0))
+    L.ftmi_set_tpr_bias(bias)
+    return int(autotune(solver, caps=tuple(candidates), tpr_biases=(bias,), replays=replays, cache=cache)["cap"])
+
+
+__all__ = ["concatenate_meshes", "CavitySpec", "StepSpec", "sample_cavities", "sample_cases", "BatchSolver",
+           "autotune", "autotune_grid_cap", "BatchJob", "plan_batch", "run_batch"]

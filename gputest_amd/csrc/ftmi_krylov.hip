@@ -683,7 +683,7 @@ struct Level {
     int64_t n;
     CsrOp<T> A, P, R;
     int tpr_a, tpr_p, tpr_r;  // threads per row of the level's CSR kernels
-    int64_t nnz_p;
+    int64_t nnz_a, nnz_p;
     const T* Dinv;
     T *b, *x, *y, *r;  // level vectors (level 0: b is the caller's rhs, x its output)
 };
@@ -708,12 +708,16 @@ struct Amg : AmgBase {
 // Rows of the coarse Galerkin levels grow long (hundreds of entries on the last levels), so
 // each CSR kernel of the cycle gives TPR lanes to a row and reduces them with width-TPR
 // shuffles; TPR = 1 keeps the plain column-order sum of the fine levels.
+// g_tpr_bias (ftmi_set_tpr_bias, the autotuner's knob) shifts every choice by that many steps
+// of 1 / 4 / 16 / 64; it is read when a level's choice is made (add_level, ftmi_amg_retune).
+static int g_tpr_bias = 0;
+
 inline int pick_tpr(int64_t n, int64_t nnz) {
     const double avg = n > 0 ? (double)nnz / (double)n : 0.0;
-    if (avg <= 8.0) return 1;
-    if (avg <= 24.0) return 4;
-    if (avg <= 96.0) return 16;
-    return 64;
+    int k = avg <= 8.0 ? 0 : avg <= 24.0 ? 1 : avg <= 96.0 ? 2 : 3;
+    k += g_tpr_bias;
+    k = k < 0 ? 0 : k > 3 ? 3 : k;
+    return 1 << (2 * k);
 }
 
 template <int TPR>
@@ -993,6 +997,7 @@ int amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col
     }
     L.tpr_a = pick_tpr(n, nnz_a);
     L.tpr_p = pick_tpr(n, nnz_p);
+    L.nnz_a = nnz_a;
     L.nnz_p = nnz_p;
     L.tpr_r = 1;  // R = P^T has the next level's rows: set when that size is known
     if (M->nlev >= 2) M->lv[M->nlev - 2].tpr_r = pick_tpr(n, M->lv[M->nlev - 2].nnz_p);
@@ -1014,6 +1019,18 @@ int amg_set_coarse(void* h, int64_t n, const T* inv, T* vb, T* vx) {
     M->cb = vb;
     M->cx = vx;
     return 0;
+}
+
+// the threads-per-row choices of every level again, with the current bias
+template <typename T>
+void amg_retune(Amg<T>* M) {
+    for (int l = 0; l < M->nlev; ++l) {
+        Level<T>& L = M->lv[l];
+        L.tpr_a = pick_tpr(L.n, L.nnz_a);
+        L.tpr_p = pick_tpr(L.n, L.nnz_p);
+        const int64_t rows_r = l + 1 < M->nlev ? M->lv[l + 1].n : M->nc;
+        L.tpr_r = pick_tpr(rows_r, L.nnz_p);
+    }
 }
 
 template <typename T>
@@ -1042,6 +1059,26 @@ int amg_vcycle(void* h, const T* b, T* x, void* stream) {
 }  // namespace
 
 extern "C" {
+
+int ftmi_set_tpr_bias(int bias) {
+    const int was = g_tpr_bias;
+    g_tpr_bias = bias < -3 ? -3 : bias > 3 ? 3 : bias;
+    return was;
+}
+
+int ftmi_amg_retune(void* h) {
+    AmgBase* b = (AmgBase*)h;
+    if (!b) {
+        set_error_msg("ftmi_amg_retune: null hierarchy");
+        return -1;
+    }
+    if (b->bits == 64)
+        amg_retune((Amg<double>*)b);
+    else
+        amg_retune((Amg<float>*)b);
+    return 0;
+}
+
 
 void* ftmi_amg_create(double omega, int sweeps) {
     Amg<double>* M = new Amg<double>();

@@ -91,8 +91,9 @@ class SimpleSolver:
         # each recorded solve runs at most ceil(a * m) + b iterations, m the largest count of
         # that solve since the previous recording (graph_cap = (a, b)); a replay in which a
         # solve did not finish within its cap is redone eagerly (graph_redos counts them)
-        self.graph_cap = (1.25, 4)
+        self.graph_cap = (1.5, 8)
         self.graph_redos = 0
+        self.graph_record_seconds = 0.0
         self._seen: list = []
         self._saved = None
         self._pin = None
@@ -321,6 +322,7 @@ class SimpleSolver:
             t.copy_(self._saved[k])
 
     def _record_graph(self):
+        t0 = time.perf_counter()
         L = nat.lib()
         a, b = self.graph_cap
         caps = [min(int(np.ceil(a * m)) + int(b), 1 << 30) for m in self._seen]
@@ -346,6 +348,7 @@ class SimpleSolver:
             raise RuntimeError(f"graph recording: {used} capped solves enqueued, {expect} expected")
         self._graph = g
         self.graph_captures += 1
+        self.graph_record_seconds += time.perf_counter() - t0
 
     def _solve_list(self, infos, pinfo, tinfo) -> list:
         """The iteration's solves in the order they are enqueued (the capture plan's slots)."""

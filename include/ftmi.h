@@ -187,6 +187,12 @@ int ftmi_gather_f64(int64_t n, const int64_t* idx, const double* x, double* y, v
  * owned device buffers of the level's size. */
 void* ftmi_amg_create(double omega, int sweeps);
 void ftmi_amg_destroy(void* amg);
+/* new: shift (-3..3) of the V-cycle's threads-per-row choice (1, 4, 16 or 64 lanes per row
+ * from the level's mean row length), applied to levels added afterwards and by
+ * ftmi_amg_retune; 0 (the default) keeps the row-length rule. Returns the previous shift. */
+int ftmi_set_tpr_bias(int bias);
+/* new: re-pick the threads per row of every level of hierarchy h with the current shift */
+int ftmi_amg_retune(void* h);
 int ftmi_amg_add_level(void* amg, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const double* a_val,
                        const int32_t* p_ptr, const int32_t* p_col, const double* p_val, const int32_t* r_ptr,
                        const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
