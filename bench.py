@@ -406,6 +406,7 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
     s.amg_rebuilds = 0
     s.amg_refreshes = 0
     s.amg_timing = {}
+    s.amg_levels_kept = []
     guard0 = len(s.guard_events)
     pits, per_step = [], []
     torch.cuda.synchronize()
@@ -424,7 +425,8 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
     info = {"stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
             "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
             "amg_refreshes": s.amg_refreshes,
-            "amg_setup_host_seconds": {k: round(v, 4) for k, v in s.amg_timing.items()},
+            "amg_setup_host_seconds": {k: round(v, 4) for k, v in s.amg_timing.items() if k != "levels_kept"},
+            "amg_levels_kept": list(s.amg_levels_kept),
             "guard_events": guards, "pressure_iters_total": int(sum(p or 0 for p in pits)),
             # the reference's own timing method (median of synchronised outer iterations,
             # hardware/planner.py:312-331); `value` stays the whole-run rate

@@ -185,26 +185,53 @@ class SmoothedAggregationAMG:
     def _setup(self, A: DevCSR, host_pattern):
         import time
 
-        L, st, dev = nat.lib(), nat.stream_ptr(), self.device
-        clock = time.perf_counter
-        # host seconds per phase: waiting for the strength mask, the greedy aggregation, the rest
-        self.timing = {"wait_mask": 0.0, "aggregate": 0.0, "total": 0.0}
-        t_start = clock()
-        f64 = lambda *s: torch.empty(*s, dtype=torch.float64, device=dev)  # noqa: E731
+        dev = self.device
+        t_start = time.perf_counter()
+        self._reset_timing()
         self._ovf = torch.zeros(4, dtype=torch.int32, device=dev)
         self._bsum = torch.empty(2048, dtype=torch.int64, device=dev)
-        partials = f64(2048 * 8)
-        self.levels, self.aggregates, self.rhos, self._plans = [], [], [], []
-        self._partials = partials
-        pattern = host_pattern
+        self._partials = torch.empty(2048 * 8, dtype=torch.float64, device=dev)
+        self.levels, self.aggregates, self.rhos, self._plans, self._masks = [], [], [], [], []
+        self._grow(A, host_pattern)
+        self._check_overflow()
+        cinv = self._coarse_inverse()
+        self.coarse_n = cinv.shape[0]
+        self.coarse_inv_h = cinv
+        self._upload(cinv)
+        self.timing["total"] = time.perf_counter() - t_start
+
+    def _reset_timing(self):
+        # host seconds per phase: waiting for the strength mask, the greedy aggregation, the
+        # whole set-up; levels whose aggregates a rebuild kept (equal strength masks)
+        self.timing = {"wait_mask": 0.0, "aggregate": 0.0, "total": 0.0, "levels_kept": 0}
+
+    def _strength_mask(self, A: DevCSR):
+        """D, D^-1 and the strength mask (one byte per entry of A) of level matrix A."""
+        L, st = nat.lib(), nat.stream_ptr()
+        n = A.n
+        diag = torch.empty(n, dtype=torch.float64, device=self.device)
+        dinv = torch.empty(n, dtype=torch.float64, device=self.device)
+        nat.check(L.ftmi_amgs_diag(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
+                                   nat.ptr(dinv), st), "diag")
+        mask = torch.empty(A.nnz, dtype=torch.uint8, device=self.device)
+        nat.check(L.ftmi_amgs_strength(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
+                                       float(self.theta), nat.ptr(mask), st), "strength")
+        return diag, dinv, mask
+
+    def _grow(self, A: DevCSR, pattern, first=None):
+        """Add levels below the current ones, starting from level matrix A (whose host pattern
+        is ``pattern`` when known), until the coarsening rule stops; A ends as the coarse
+        matrix. ``first`` = (diag, dinv, mask) of A when already computed."""
+        import time
+
+        L, st, dev = nat.lib(), nat.stream_ptr(), self.device
+        clock = time.perf_counter
+        f64 = lambda *s: torch.empty(*s, dtype=torch.float64, device=dev)  # noqa: E731
+        partials = self._partials
         while A.n > self.max_coarse and len(self.levels) < self.max_levels - 1:
             n = A.n
-            diag, dinv = f64(n), f64(n)
-            nat.check(L.ftmi_amgs_diag(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
-                                       nat.ptr(dinv), st), "diag")
-            mask = torch.empty(A.nnz, dtype=torch.uint8, device=dev)
-            nat.check(L.ftmi_amgs_strength(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
-                                           float(self.theta), nat.ptr(mask), st), "strength")
+            diag, dinv, mask = first if first is not None else self._strength_mask(A)
+            first = None
             # the strength mask (and a coarse level's pattern) go to pinned host memory while
             # the GPU runs the power iteration, which does not depend on the aggregates
             mask_p = _pinned(A.nnz, torch.uint8)
@@ -251,20 +278,71 @@ class SmoothedAggregationAMG:
             Ac = self._spgemm(R, AP, keep=plan["kRAP"])
             plan["AP"] = AP
             self._plans.append(plan)
+            self._masks.append(mask)
             self.levels.append((A, P, R, dinv))
             self.aggregates.append(agg_h.astype(np.int64))  # a copy: the pinned buffer is recycled
             self.rhos.append(rho)
             A, pattern = Ac, None
+        self.coarse = A
+
+    def _check_overflow(self):
         ovf = self._ovf.cpu().numpy()
         if ovf[0]:
             raise RuntimeError(f"AMG set-up: {ovf[0]} sparse-product rows exceeded the LDS hash capacity "
                                f"(max distinct {ovf[1]}, max candidates {ovf[2]}, e.g. row {ovf[3]})")
-        self.coarse = A
+
+    def rebuild_ldu(self, fv, diag, off):
+        """The hierarchy of a new pressure matrix (face form of ``fv``), equal to a full set-up
+        of it (``from_ldu``) value for value: the greedy aggregation is a function of the level's
+        pattern and strength mask alone, so wherever a level's new strength mask equals the one
+        its aggregates came from, the aggregates (and, below them, every sparsity pattern) are
+        those a fresh aggregation would give, and the level's values are recomputed on the kept
+        patterns as ``refresh`` does. From the first level whose mask differs, the levels are
+        set up anew. Needs a hierarchy set up with ``refreshable=True``."""
+        import time
+
+        if not self.refreshable:
+            raise RuntimeError("this hierarchy was set up without refreshable=True")
+        t_start = time.perf_counter()
+        self._reset_timing()
+        L, st = nat.lib(), nat.stream_ptr()
+        A0 = self.levels[0][0] if self.levels else self.coarse
+        nat.check(L.ftmi_amgs_ldu_to_csr(fv.nc, nat.ptr(fv.cptr), nat.ptr(fv.he_slot), nat.ptr(fv.slot_diag),
+                                         nat.ptr(fv.indptr), nat.ptr(diag), nat.ptr(off), nat.ptr(A0.val), st),
+                  "ldu_to_csr")
+        nlev = len(self.levels)
+        # The finest level's mask is checked first (one host read), since refreshing the whole
+        # hierarchy for nothing costs most when it changes. The coarser levels are then refreshed
+        # on their kept patterns while their new masks are compared with the kept ones on the
+        # device (one host read for all of them); a level below the first changed mask has then
+        # received values computed on stale aggregates, and is set up anew below.
+        firsts, flags = [], []
+        for k in range(nlev):
+            first = self._strength_mask(self.levels[k][0])
+            firsts.append(first)
+            flags.append(torch.ne(first[2], self._masks[k]).any())
+            if k == 0 and bool(flags[0]):
+                break
+            self._refresh_level(k, first)
+        flags = torch.stack(flags).cpu().tolist() if flags else []
+        k = flags.index(True) if True in flags else nlev
+        first = firsts[k] if k < nlev else None
+        if k < nlev or self.coarse.n > self.max_coarse:
+            # new aggregates from level k on (or the coarse matrix may now coarsen further)
+            A = self.levels[k][0] if k < nlev else self.coarse
+            del self.levels[k:], self._plans[k:], self._masks[k:], self.aggregates[k:], self.rhos[k:]
+            self._grow(A, (fv.indptr_h, fv.indices_h) if k == 0 else None, first)
+            self._check_overflow()
+        self.timing["levels_kept"] = k
         cinv = self._coarse_inverse()
-        self.coarse_n = cinv.shape[0]
         self.coarse_inv_h = cinv
-        self._upload(cinv)
-        self.timing["total"] = clock() - t_start
+        if k == nlev and len(self.levels) == nlev and cinv.shape[0] == self.coarse_n:
+            self._cinv.copy_(torch.as_tensor(np.ascontiguousarray(cinv), device=self.device))
+        else:
+            self.coarse_n = cinv.shape[0]
+            self._destroy_handle()
+            self._upload(cinv)
+        self.timing["total"] = time.perf_counter() - t_start
 
     def _coarse_inverse(self) -> np.ndarray:
         Ad = self.coarse.to_scipy().toarray()
@@ -298,30 +376,35 @@ class SmoothedAggregationAMG:
         if not self.refreshable:
             raise RuntimeError("this hierarchy was set up without refreshable=True")
         L, st = nat.lib(), nat.stream_ptr()
-        for (A, P, R, dinv), plan in zip(self.levels, self._plans):
-            n = A.n
-            nat.check(L.ftmi_amgs_diag(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(plan["diag"]),
-                                       nat.ptr(dinv), st), "diag")
-            x = start_vector_device(n, self.device)
-            nat.check(L.ftmi_amgs_power(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv),
-                                        nat.ptr(x), nat.ptr(plan["y"]), 15, nat.ptr(self._partials),
-                                        nat.ptr(plan["rho"]), st), "power")
-            T = plan["T"]
-            self._spgemm_fill(A, T, P, plan["kP"], xscale=dinv, agg=T.col, tval=T.val, omega_num=plan["rho"],
-                              omega_scale=4.0 / 3.0)
-            nat.check(L.ftmi_gather_f64(R.nnz, nat.ptr(plan["perm"]), nat.ptr(P.val), nat.ptr(R.val), st), "gather")
-            self._spgemm_fill(A, P, plan["AP"], plan["kAP"])
-            nxt = self._plans_next(A)
-            self._spgemm_fill(R, plan["AP"], nxt, plan["kRAP"])
+        for k, ((A, P, R, dinv), plan) in enumerate(zip(self.levels, self._plans)):
+            nat.check(L.ftmi_amgs_diag(A.n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val),
+                                       nat.ptr(plan["diag"]), nat.ptr(dinv), st), "diag")
+            self._refresh_level(k)
         cinv = self._coarse_inverse()
         self.coarse_inv_h = cinv
         self._cinv.copy_(torch.as_tensor(np.ascontiguousarray(cinv), device=self.device))
 
-    def _plans_next(self, A):
-        for k, lv in enumerate(self.levels):
-            if lv[0] is A:
-                return self.levels[k + 1][0] if k + 1 < len(self.levels) else self.coarse
-        raise KeyError("level not found")
+    def _refresh_level(self, k, first=None):
+        """Values of level k's P, R, A P and of the next level's matrix from level k's current
+        values, on the kept patterns. D and D^-1 are ``first``[:2] when given (moved into the
+        level's buffers), else already current."""
+        L, st = nat.lib(), nat.stream_ptr()
+        A, P, R, dinv = self.levels[k]
+        plan = self._plans[k]
+        if first is not None:
+            plan["diag"].copy_(first[0])
+            dinv.copy_(first[1])
+        x = start_vector_device(A.n, self.device)
+        nat.check(L.ftmi_amgs_power(A.n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(dinv),
+                                    nat.ptr(x), nat.ptr(plan["y"]), 15, nat.ptr(self._partials),
+                                    nat.ptr(plan["rho"]), st), "power")
+        T = plan["T"]
+        self._spgemm_fill(A, T, P, plan["kP"], xscale=dinv, agg=T.col, tval=T.val, omega_num=plan["rho"],
+                          omega_scale=4.0 / 3.0)
+        nat.check(L.ftmi_gather_f64(R.nnz, nat.ptr(plan["perm"]), nat.ptr(P.val), nat.ptr(R.val), st), "gather")
+        self._spgemm_fill(A, P, plan["AP"], plan["kAP"])
+        nxt = self.levels[k + 1][0] if k + 1 < len(self.levels) else self.coarse
+        self._spgemm_fill(R, plan["AP"], nxt, plan["kRAP"])
 
     def refresh_ldu(self, fv, diag, off):
         """``refresh`` for a new pressure matrix given in the face (LDU) form of ``fv``."""
@@ -371,7 +454,7 @@ class SmoothedAggregationAMG:
         nat.check(nat.lib().ftmi_amg_vcycle(self.handle, nat.ptr(b), nat.ptr(x), nat.stream_ptr()), "amg_vcycle")
         return x
 
-    def __del__(self):
+    def _destroy_handle(self):
         h = getattr(self, "handle", None)
         if h:
             try:
@@ -379,6 +462,9 @@ class SmoothedAggregationAMG:
             except Exception:
                 pass
             self.handle = None
+
+    def __del__(self):
+        self._destroy_handle()
 
 
 class AmgFP32:

@@ -30,7 +30,7 @@ from __future__ import annotations
 import ctypes
 import time
 from collections import defaultdict
-from typing import Dict
+from typing import Dict, List
 
 import numpy as np
 import torch
@@ -114,6 +114,7 @@ class SimpleSolver:
         self.amg_refreshes = 0
         self._amg_refreshed = 0
         self.amg_timing: Dict[str, float] = {}
+        self.amg_levels_kept: List[int] = []  # per rebuild: levels whose aggregates stayed
         self.last_linear = {}
         self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
         self._build_boundaries()
@@ -471,12 +472,16 @@ class SimpleSolver:
                 self._amg_refreshed += 1
                 self.amg_refreshes += 1
             else:
-                self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p,
-                                                           refreshable=self.mixed)
+                if self._amg is None:
+                    self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp,
+                                                               singular=self.singular_p, refreshable=True)
+                else:  # the hierarchy a full set-up gives, keeping the levels whose aggregates stay
+                    self._amg.rebuild_ldu(fv, self.diag_p, self.offp)
                 self._amg32 = self._amg.to_fp32() if self.mixed else None
                 self._amg_refreshed = 0
                 for k, v in self._amg.timing.items():
                     self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
+                self.amg_levels_kept.append(int(self._amg.timing["levels_kept"]))
                 self.amg_rebuilds += 1
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
