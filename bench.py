@@ -59,10 +59,12 @@ def _workload(nx, ny, precision="fp64"):
 
 def bi_pv_bytes(n: int, nf: int) -> int:
     """Compulsory HBM bytes of one launch of the BiCGSTAB p-update + SpMV kernel on the face
-    (LDU) operator: reads r, p_prev, v_prev, inv, rhat, diag (6 n doubles), the adjacency
-    pointers (n + 1 int32), neighbours (2 nf int32) and coefficients (2 nf doubles); writes
-    p, phat, v (3 n doubles). Neighbour values gathered again count once."""
-    return int(n * 8 * 6 + (n + 1) * 4 + 2 * nf * (4 + 8) + n * 8 * 3)
+    operator in its 64-row slice layout: reads r, p_prev, v_prev, inv, rhat, diag (6 n
+    doubles), the slice bases (n / 64 int32) and row degrees (n bytes), neighbours (2 nf
+    int32) and coefficients (2 nf doubles); writes p and v (2 n doubles). Neighbour values
+    gathered again count once; the previous iteration's partial sums (read from L2 by every
+    workgroup) are not counted."""
+    return int(n * 8 * 6 + (n + 63) // 64 * 4 + n + 2 * nf * (4 + 8) + n * 8 * 2)
 
 
 def measure_roofline(s, steps=3):

@@ -212,12 +212,14 @@ static double* capture_slot(int64_t maxiter, int64_t* cap) {
 }
 
 // ------------------------------------------------------------------ BiCGSTAB kernels
-// One iteration is four launches: p-update fused into v = A phat (phat recomputed at the
-// neighbours from r, p, v of the previous iteration), s-update fused into t = A shat, the
-// residual/iterate update, and the state finalize that the host polls. p and v alternate
-// between two buffers so that no kernel reads what it writes. The scalars of a reduction are
-// computed by every workgroup of the kernel that consumes it (consume_sum), so only the
-// end-of-iteration state needs its own one-workgroup kernel.
+// One iteration is three launches: the p-update fused into v = A phat (phat recomputed at the
+// neighbours from r, p, v of the previous iteration), the s-update fused into t = A shat, and
+// the residual/iterate update. The scalars of a reduction are computed by every workgroup of
+// the kernel that consumes it (consume_sum); the end-of-iteration bookkeeping (norms, flags,
+// the host post) is done by the next iteration's first kernel, whose workgroups all take the
+// same decisions from the same partials and of which workgroup 0 writes the state. phat and
+// shat are recomputed as inv p and inv s where they are needed again (bit-identical to
+// storing them), so the iteration moves two vectors fewer.
 // ZERO: the start vector is zero (x is cleared and r = b without the product)
 template <class Op, typename T, bool ZERO>
 __global__ void __launch_bounds__(kBlock) bi_init(int64_t n, Op A, const T* __restrict__ b, T* __restrict__ x,
@@ -270,16 +272,85 @@ __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, in
     post_host(S, post);
 }
 
-// p = r + beta (p_prev - omega v_prev), phat = inv p, v = A phat; partials of rhat . v
-template <class Op, typename T>
+// End of iteration k from the totals (r.r, rhat.r) of its residual update: the iterate
+// becomes current when ||r|| is finite (else the solve stops on the last finite one), then
+// the convergence test, omega = 0 and rho = 0 breakdowns. Every thread that calls it gets the
+// same answer (true: the solve is done); only `lead` writes the state block.
+__device__ inline bool bi_finish_iteration(State* S, const double (&tot)[2], double tol, int k, int maxiter,
+                                           bool lead) {
+    double* sc = S->sc;
+    int* st = S->st;
+    const double rnorm = sqrt(tot[0]);
+    if (!isfinite(rnorm)) {  // breakdown: the current iterate stays
+        if (lead) {
+            st[I_DONE] = 1;
+            st[I_CONV] = 0;
+            st[I_ITERS] = k;
+            sc[S_RES] = __builtin_nan("");
+        }
+        return true;
+    }
+    const double bnorm = sc[S_BNORM];
+    if (lead) {
+        st[I_CUR] ^= 1;
+        sc[S_RNORM] = rnorm;
+        sc[S_RES] = rnorm / bnorm;
+    }
+    int done = 0, conv = 0, iters = k;
+    if (rnorm / bnorm < tol || rnorm <= 1e-14 * sc[S_BABS]) {
+        done = conv = 1;
+    } else if (sc[S_OMEGA] == 0.0) {
+        done = 1;
+    } else {
+        if (lead) sc[S_RHONEW] = tot[1];
+        if (tot[1] == 0.0 && k < maxiter) {  // the next iteration breaks at its first step
+            done = 1;
+            iters = k + 1;
+        }
+    }
+    if (done && lead) {
+        st[I_DONE] = 1;
+        st[I_CONV] = conv;
+        st[I_ITERS] = iters;
+    }
+    return done != 0;
+}
+
+// The end of iteration k as a kernel of its own (after the last enqueued iteration, and in
+// capture mode's tail).
+__global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int nparts, State* S, double tol, int k,
+                                                   int maxiter, Post post) {
+    double tot[2];
+    if (!S->st[I_DONE] && finalize_sum<2>(partials, nparts, tot)) bi_finish_iteration(S, tot, tol, k, maxiter, true);
+    post_host(S, post);
+}
+
+// FIN: first finish iteration k - 1 from the partials of its residual update (the work of
+// fin_bi_r, posting the state to the host), then, unless done:
+// p = r + beta (p_prev - omega v_prev), v = A (inv p); partials of rhat . v
+template <class Op, typename T, bool FIN>
 __global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const T* __restrict__ r,
                                                 const T* __restrict__ p_prev, const T* __restrict__ v_prev,
                                                 const T* __restrict__ inv, const T* __restrict__ rh,
-                                                T* __restrict__ p, T* __restrict__ ph, T* __restrict__ v,
-                                                const State* S, double* part_out) {
-    if (S->st[I_DONE]) return;
+                                                T* __restrict__ p, T* __restrict__ v, State* S,
+                                                const double* part_in, int np_in, double tol, int k, int maxiter,
+                                                Post post, double* part_out) {
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (S->st[I_DONE]) {
+        if (FIN && lead) post_host(S, post);
+        return;
+    }
     const double* sc = S->sc;
-    const T beta = (T)((sc[S_RHONEW] / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]));
+    double rhonew = sc[S_RHONEW];
+    if (FIN) {
+        double tot[2];
+        consume_sum<2>(part_in, np_in, tot);
+        const bool done = bi_finish_iteration(S, tot, tol, k - 1, maxiter, lead);
+        if (lead) post_host(S, post);
+        if (done) return;
+        rhonew = tot[1];
+    }
+    const T beta = (T)((rhonew / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]));
     const T om = (T)sc[S_OMEGA];
     auto phat = [&](int64_t j) { return inv[j] * (r[j] + beta * (p_prev[j] - om * v_prev[j])); };
     int64_t lo, hi;
@@ -288,7 +359,6 @@ __global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const T* __rest
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         const T pi = r[i] + beta * (p_prev[i] - om * v_prev[i]);
         p[i] = pi;
-        ph[i] = inv[i] * pi;
         const T vi = A.row_f(i, phat);
         v[i] = vi;
         acc[0] += rh[i] * vi;
@@ -296,12 +366,11 @@ __global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const T* __rest
     block_partials<1>(acc, part_out);
 }
 
-// alpha = rho / (rhat . v); s = r - alpha v, shat = inv s, t = A shat; partials s.s, t.t, t.s
+// alpha = rho / (rhat . v); s = r - alpha v, t = A (inv s); partials s.s, t.t, t.s
 template <class Op, typename T>
 __global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const T* __restrict__ r, const T* __restrict__ v,
-                                                const T* __restrict__ inv, T* __restrict__ s, T* __restrict__ sh,
-                                                T* __restrict__ t, State* S, const double* part_in, int np_in,
-                                                double* part_out, int k) {
+                                                const T* __restrict__ inv, T* __restrict__ s, T* __restrict__ t,
+                                                State* S, const double* part_in, int np_in, double* part_out, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
     consume_sum<1>(part_in, np_in, tot);
@@ -330,7 +399,6 @@ __global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const T* __rest
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         const T si = r[i] - alpha * v[i];
         s[i] = si;
-        sh[i] = inv[i] * si;
         const T ti = A.row_f(i, shat);
         t[i] = ti;
         acc[0] += si * si;
@@ -340,10 +408,11 @@ __global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const T* __rest
     block_partials<3>(acc, part_out);
 }
 
-// early exit on ||s||; omega = (t.s)/(t.t); r = s - omega t; x_other = x_cur + alpha phat + omega shat
+// early exit on ||s||; omega = (t.s)/(t.t); r = s - omega t;
+// x_other = x_cur + alpha (inv p) + omega (inv s)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const T* __restrict__ s, const T* __restrict__ t,
-                                               const T* __restrict__ ph, const T* __restrict__ sh,
+                                               const T* __restrict__ p, const T* __restrict__ inv,
                                                const T* __restrict__ rh, T* __restrict__ r, T* __restrict__ x0,
                                                T* __restrict__ x1, State* S, const double* part_in, int np_in,
                                                double* part_out, double tol, int k) {
@@ -373,96 +442,64 @@ __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const T* __restrict__ 
     block_range(n, lo, hi);
     T acc[2] = {T(0), T(0)};
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const T ri = s[i] - om * t[i];
+        const T si = s[i];
+        const T ri = si - om * t[i];
         r[i] = ri;
-        xo[i] = xc[i] + alpha * ph[i] + om * sh[i];
+        const T wi = inv[i];
+        xo[i] = xc[i] + alpha * (wi * p[i]) + om * (wi * si);
         acc[0] += ri * ri;
         acc[1] += rh[i] * ri;
     }
     block_partials<2>(acc, part_out);
 }
 
-__global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int nparts, State* S, double tol, int k,
-                                                   int maxiter, Post post) {
-    double tot[2];
-    if (!S->st[I_DONE] && finalize_sum<2>(partials, nparts, tot)) {
-        double* sc = S->sc;
-        int* st = S->st;
-        const int cur = st[I_CUR];
-        const double om = sc[S_OMEGA];
-        const double rnorm = sqrt(tot[0]);
-        if (!isfinite(rnorm)) {  // breakdown: the current iterate stays
-            st[I_DONE] = 1;
-            st[I_CONV] = 0;
-            st[I_ITERS] = k;
-            sc[S_RES] = __builtin_nan("");
-        } else {
-            st[I_CUR] = cur ^ 1;
-            sc[S_RNORM] = rnorm;
-            sc[S_RES] = rnorm / sc[S_BNORM];
-            if (rnorm / sc[S_BNORM] < tol || rnorm <= 1e-14 * sc[S_BABS]) {
-                st[I_DONE] = 1;
-                st[I_CONV] = 1;
-                st[I_ITERS] = k;
-            } else if (om == 0.0) {
-                st[I_DONE] = 1;
-                st[I_CONV] = 0;
-                st[I_ITERS] = k;
-            } else {
-                sc[S_RHONEW] = tot[1];
-                if (tot[1] == 0.0 && k < maxiter) {  // the next iteration breaks at its first step
-                    st[I_DONE] = 1;
-                    st[I_CONV] = 0;
-                    st[I_ITERS] = k + 1;
-                }
-            }
-        }
-    }
-    post_host(S, post);
-}
-
-// x0 = x_cur + alpha phat (early exit on ||s||), or x0 = x1 when x1 is current.
+// x0 = x_cur + alpha (inv p) (early exit on ||s||), or x0 = x1 when x1 is current.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) bi_final(int64_t n, T* __restrict__ x0, const T* __restrict__ x1,
-                                                   const T* __restrict__ ph, double alpha_d, int cur, int fixup) {
+                                                   const T* __restrict__ p, const T* __restrict__ inv,
+                                                   double alpha_d, int cur, int fixup) {
     const T alpha = (T)alpha_d;
     int64_t lo, hi;
     block_range(n, lo, hi);
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         const T xc = cur ? x1[i] : x0[i];
-        x0[i] = fixup ? xc + alpha * ph[i] : xc;
+        x0[i] = fixup ? xc + alpha * (inv[i] * p[i]) : xc;
     }
 }
 
-// bi_final with cur / fixup / alpha read from the state block on the device
+// bi_final with cur / fixup / alpha read from the state block on the device; p of the last
+// iteration is p_odd or p_even by the parity of the iteration count
 template <typename T>
 __global__ void __launch_bounds__(kBlock) bi_final_dev(int64_t n, T* __restrict__ x0, const T* __restrict__ x1,
-                                                       const T* __restrict__ ph, const State* S) {
+                                                       const T* __restrict__ p_even, const T* __restrict__ p_odd,
+                                                       const T* __restrict__ inv, const State* S) {
     const int cur = S->st[I_CUR], fixup = S->st[I_FIXUP];
     if (!cur && !fixup) return;
     const T alpha = (T)S->sc[S_ALPHA];
+    const T* p = (S->st[I_ITERS] & 1) ? p_odd : p_even;
     int64_t lo, hi;
     block_range(n, lo, hi);
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
         const T xc = cur ? x1[i] : x0[i];
-        x0[i] = fixup ? xc + alpha * ph[i] : xc;
+        x0[i] = fixup ? xc + alpha * (inv[i] * p[i]) : xc;
     }
 }
 
 template <typename T>
 struct BiWork {
-    T *r, *rh, *p[2], *v[2], *ph, *s, *sh, *t, *x1;
+    T *r, *rh, *p[2], *v[2], *s, *t, *x1;
     State* S;
     unsigned* counter;
-    double* part[2];
+    double* part[3];
 };
 
 constexpr int64_t kPartBytes = (int64_t)kRedGrid * kMaxRed * 8;
 inline int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+constexpr int kBiVectors = 9;
 
 template <typename T>
 static int64_t bi_bytes(int64_t n) {
-    return align256(11 * n * (int64_t)sizeof(T)) + 512 + 2 * kPartBytes;
+    return align256(kBiVectors * n * (int64_t)sizeof(T)) + 512 + 3 * kPartBytes;
 }
 
 template <typename T>
@@ -476,22 +513,50 @@ static BiWork<T> carve_bi(int64_t n, void* work) {
     W.p[1] = d + 3 * n;
     W.v[0] = d + 4 * n;
     W.v[1] = d + 5 * n;
-    W.ph = d + 6 * n;
-    W.s = d + 7 * n;
-    W.sh = d + 8 * n;
-    W.t = d + 9 * n;
-    W.x1 = d + 10 * n;
-    char* tail = w + align256(11 * n * (int64_t)sizeof(T));
+    W.s = d + 6 * n;
+    W.t = d + 7 * n;
+    W.x1 = d + 8 * n;
+    char* tail = w + align256(kBiVectors * n * (int64_t)sizeof(T));
     W.S = (State*)tail;
     W.counter = (unsigned*)(tail + 256);
-    W.part[0] = (double*)(tail + 512);
-    W.part[1] = (double*)(tail + 512 + kPartBytes);
+    for (int j = 0; j < 3; ++j) W.part[j] = (double*)(tail + 512 + j * kPartBytes);
     return W;
 }
 
 // the mixed-precision transport solve keeps its FP32 right-hand side and correction in
 // front of an FP32 BiCGSTAB workspace
 static int64_t bi_mixed_bytes(int64_t n) { return align256(2 * n * (int64_t)sizeof(float)) + bi_bytes<float>(n); }
+
+// Enqueue iteration k: bi_pv (finishing iteration k - 1 when k > 1, posting its state to
+// `post`), bi_st, bi_r. Partials: bi_r -> part[2] -> bi_pv -> part[0] -> bi_st -> part[1]
+// -> bi_r, so no kernel reads a buffer it writes.
+template <class Op, typename T>
+static void bi_iteration(int64_t n, Op A, const T* inv, T* x, BiWork<T>& W, int g, double tol, int64_t k,
+                         int64_t maxiter, Post post, hipStream_t st, hipEvent_t t0 = nullptr,
+                         hipEvent_t t1 = nullptr) {
+    const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+    const T* r = W.r;
+    if (k > 1) {
+        if (t0)  // events carried by the dispatch: the kernel's own start and end
+            hipExtLaunchKernelGGL(bi_pv<Op, T, true>, dim3(g), dim3(kBlock), 0, st, t0, t1, 0, n, A, r,
+                                  (const T*)W.p[a], (const T*)W.v[a], inv, (const T*)W.rh, W.p[c], W.v[c], W.S,
+                                  (const double*)W.part[2], g, tol, (int)k, (int)maxiter, post, W.part[0]);
+        else
+            bi_pv<Op, T, true><<<g, kBlock, 0, st>>>(n, A, r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.v[c], W.S,
+                                                     W.part[2], g, tol, (int)k, (int)maxiter, post, W.part[0]);
+    } else {
+        if (t0)
+            hipExtLaunchKernelGGL(bi_pv<Op, T, false>, dim3(g), dim3(kBlock), 0, st, t0, t1, 0, n, A, r,
+                                  (const T*)W.p[a], (const T*)W.v[a], inv, (const T*)W.rh, W.p[c], W.v[c], W.S,
+                                  (const double*)nullptr, 0, tol, (int)k, (int)maxiter, kNoPost, W.part[0]);
+        else
+            bi_pv<Op, T, false><<<g, kBlock, 0, st>>>(n, A, r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.v[c], W.S,
+                                                      nullptr, 0, tol, (int)k, (int)maxiter, kNoPost, W.part[0]);
+    }
+    bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.t, W.S, W.part[0], g, W.part[1], (int)k);
+    bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.p[c], inv, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[2], tol,
+                                  (int)k);
+}
 
 }  // namespace
 
@@ -524,55 +589,45 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
     if (g_capture) {
         int64_t cap;
         double* report = capture_slot(maxiter, &cap);
-        for (int64_t k = 1; k <= cap; ++k) {
-            const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-            bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
-                                               W.part[0]);
-            bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
-                                               (int)k);
-            bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
-                                          tol, (int)k);
-            fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, kNoPost);
-        }
-        bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, W.S);
+        for (int64_t k = 1; k <= cap; ++k) bi_iteration<Op, T>(n, A, inv, x, W, g, tol, k, maxiter, kNoPost, st);
+        if (cap >= 1) fin_bi_r<<<1, kBlock, 0, st>>>(W.part[2], g, W.S, tol, (int)cap, (int)maxiter, kNoPost);
+        bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.p[0], W.p[1], inv, W.S);
         if (report) k_report<<<1, 64, 0, st>>>(W.S, report, cap >= maxiter ? 1 : 0, cap);
         FTMI_LAUNCH_CHECK("bicgstab (capture)");
         *iters = -1;
         *res = __builtin_nan("");
         return 0;
     }
-    // slot `last` holds the most recently posted state; the loop waits one iteration behind
-    int last = 0;
-    for (int64_t k = 1; k <= maxiter; ++k) {
-        const int a = (int)((k - 1) & 1), c = (int)(k & 1);
+    // The state after iteration j goes to host slot j & 1 (j = 0: fin_bi_init). Iteration k's
+    // first kernel posts the state after k - 1; the host then waits for the one after k - 2,
+    // so the GPU always has the rest of an iteration queued while the host reads.
+    int64_t k = 1;
+    for (; k <= maxiter; ++k) {
+        const Post post = k > 1 ? next_post(pn, (int)((k - 1) & 1)) : kNoPost;
         hipEvent_t t0 = nullptr, t1 = nullptr;
         if (g_timer.on) {
-            // events carried by the dispatch itself: the kernel's own start and end, without
-            // the dispatch latency that separate event records around the launch would add
             t0 = g_timer.get();
             t1 = g_timer.get();
-            hipExtLaunchKernelGGL(bi_pv<Op, T>, dim3(g), dim3(kBlock), 0, st, t0, t1, 0, n, A, (const T*)W.r,
-                                  (const T*)W.p[a], (const T*)W.v[a], inv, (const T*)W.rh, W.p[c], W.ph, W.v[c],
-                                  (const State*)W.S, W.part[0]);
             g_timer.pending.emplace_back(t0, t1);
-        } else {
-            bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
-                                               W.part[0]);
         }
-        bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
-                                           (int)k);
-        bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
-                                      tol, (int)k);
-        fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, next_post(pn, last ^ 1));
+        bi_iteration<Op, T>(n, A, inv, x, W, g, tol, k, maxiter, post, st, t0, t1);
         FTMI_LAUNCH_CHECK("bicgstab iteration");
-        if (wait_state(pn, last, st)) return -1;  // the previous iteration's flags
-        const bool prev_done = pn->h[last].st[I_DONE] != 0;
-        last ^= 1;
-        if (prev_done) break;
+        if (k >= 2) {
+            const int slot = (int)(k & 1);  // the state after iteration k - 2
+            if (wait_state(pn, slot, st)) return -1;
+            if (pn->h[slot].st[I_DONE]) {
+                ++k;
+                break;
+            }
+        }
     }
-    // every posted state is final once the solve is done; the last one is the latest
-    if (wait_state(pn, last, st)) return -1;
-    const int slot = last;
+    const int64_t last = k - 1;  // the last iteration enqueued (0: none, the state is fin_bi_init's)
+    const int slot = (int)(last & 1);
+    if (last > 0) {
+        fin_bi_r<<<1, kBlock, 0, st>>>(W.part[2], g, W.S, tol, (int)last, (int)maxiter, next_post(pn, slot));
+        FTMI_LAUNCH_CHECK("bicgstab finish");
+    }
+    if (wait_state(pn, slot, st)) return -1;
     const State& H = pn->h[slot];
     int conv;
     if (H.st[I_DONE]) {
@@ -585,7 +640,8 @@ int bicgstab_run(int64_t n, Op A, const T* b, const T* inv, T* x, double tol, in
         conv = 0;
     }
     if (H.st[I_CUR] || H.st[I_FIXUP])
-        bi_final<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, H.sc[S_ALPHA], H.st[I_CUR], H.st[I_FIXUP]);
+        bi_final<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.p[H.st[I_ITERS] & 1], inv, H.sc[S_ALPHA],
+                                                    H.st[I_CUR], H.st[I_FIXUP]);
     FTMI_LAUNCH_CHECK("bicgstab final");
     return conv;
 }
@@ -1748,7 +1804,7 @@ int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const 
 namespace {
 
 constexpr int64_t kNoAllocRows = int64_t(1) << 18;
-constexpr int64_t kNoAllocBytes = (11 * kNoAllocRows * 8 + 255) / 256 * 256 + 512 + 2 * kPartBytes;
+constexpr int64_t kNoAllocBytes = (kBiVectors * kNoAllocRows * 8 + 255) / 256 * 256 + 512 + 3 * kPartBytes;
 constexpr int kNoAllocChunk = 8;  // iterations enqueued between two reads of the state block
 constexpr int kNoAllocDevices = 64;
 
@@ -1806,23 +1862,18 @@ int bicgstab_run_noalloc(int64_t n, Op A, const T* b, const T* inv, T* x, double
     State H;
     int64_t k = 1;
     for (;;) {
+        // the state read here has iterations up to k - 2 finished (the last one enqueued is
+        // finished by the next iteration's first kernel)
         if (int e = read_state(&H, W.S, st)) return e;
         if (H.st[I_DONE] || k > maxiter) break;
         const int64_t stop = k + kNoAllocChunk - 1 < maxiter ? k + kNoAllocChunk - 1 : maxiter;
-        for (; k <= stop; ++k) {
-            const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-            bi_pv<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.p[a], W.v[a], inv, W.rh, W.p[c], W.ph, W.v[c], W.S,
-                                               W.part[0]);
-            bi_st<Op, T><<<g, kBlock, 0, st>>>(n, A, W.r, W.v[c], inv, W.s, W.sh, W.t, W.S, W.part[0], g, W.part[1],
-                                               (int)k);
-            bi_r<T><<<g, kBlock, 0, st>>>(n, W.s, W.t, W.ph, W.sh, W.rh, W.r, x, W.x1, W.S, W.part[1], g, W.part[0],
-                                          tol, (int)k);
-            fin_bi_r<<<1, kBlock, 0, st>>>(W.part[0], g, W.S, tol, (int)k, (int)maxiter, kNoPost);
-        }
+        for (; k <= stop; ++k) bi_iteration<Op, T>(n, A, inv, x, W, g, tol, k, maxiter, kNoPost, st);
         FTMI_LAUNCH_CHECK("bicgstab_noalloc iteration");
     }
-    bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.ph, W.S);
+    if (k > 1) fin_bi_r<<<1, kBlock, 0, st>>>(W.part[2], g, W.S, tol, (int)(k - 1), (int)maxiter, kNoPost);
+    bi_final_dev<T><<<grid_for(n), kBlock, 0, st>>>(n, x, W.x1, W.p[0], W.p[1], inv, W.S);
     FTMI_LAUNCH_CHECK("bicgstab_noalloc final");
+    if (int e = read_state(&H, W.S, st)) return e;
     if (H.st[I_DONE]) {
         *iters = H.st[I_ITERS];
         *res = H.sc[S_RES];
