@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftmi.so")
-ABI_VERSION = 104
+ABI_VERSION = 105
 
 P = ctypes.c_void_p
 I32, I64, F64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -89,11 +89,11 @@ _SIGS = {
     "ftmi_pcg_csr": ([I64, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
     "ftmi_amg_create": ([F64, I32], P),
     "ftmi_amg_destroy": ([P], None),
-    "ftmi_amg_add_level": ([P, I64] + [P] * 14, I32),
+    "ftmi_amg_add_level": ([P, I64, I64, I64] + [P] * 14, I32),
     "ftmi_amg_set_coarse": ([P, I64, P, P, P], I32),
     "ftmi_amg_vcycle": ([P, P, P, P], I32),
     "ftmi_amg_create_f32": ([F64, I32], P),
-    "ftmi_amg_add_level_f32": ([P, I64] + [P] * 14, I32),
+    "ftmi_amg_add_level_f32": ([P, I64, I64, I64] + [P] * 14, I32),
     "ftmi_amg_set_coarse_f32": ([P, I64, P, P, P], I32),
     "ftmi_amg_vcycle_f32": ([P, P, P, P], I32),
     "ftmi_pcg_ldu_f32": ([I64, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
@@ -103,6 +103,9 @@ _SIGS = {
     "ftmi_gather_f64": ([I64, P, P, P, P], I32),
     "ftmi_amg_aggregate_host": ([I64, P, P, P], I64),
     "ftmi_amg_aggregate_mask_host": ([I64, P, P, P, P, P], I64),
+    "ftmi_amg_aggregate_csr_host": ([I64, P, P, P, P], I64),
+    "ftmi_amgs_strong_count": ([I64, P, P, P, P], I32),
+    "ftmi_amgs_strong_fill": ([I64, P, P, P, P, P, P], I32),
     "ftmi_diag_rows": ([I64, P, P, P, P, P, P, F64, P, P, P, P, P], I32),
     "ftmi_diag_cell_flux": ([I64, P, P, P, P, P, P, P, P, P], I32),
     "ftmi_diag_work_doubles": ([], I64),

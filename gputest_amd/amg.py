@@ -11,8 +11,10 @@ weighted-Jacobi cycles with omega = 0.8.
 
 Set-up runs on the GPU (``ftmi_amgs_*``) except the greedy aggregation, which is
 sequential by definition (its result depends on the node order) and runs on the host over
-the strength mask (one byte per matrix entry copied back per level). The coarsest dense
-inverse (at most ``max_coarse`` squared entries) is computed on the host.
+the strength graph copied back per level (the finest level's mask, one byte per entry of a
+pattern the host already holds; a coarse level's strong entries as a CSR of their own). The
+coarsest dense inverse (at most ``max_coarse`` squared entries) is computed on the host. A
+rebuild for new values (``rebuild_ldu``) keeps every level whose strength mask did not change.
 """
 from __future__ import annotations
 
@@ -169,7 +171,10 @@ class SmoothedAggregationAMG:
         nat.check(nat.lib().ftmi_scan_i32(n, nat.ptr(cnt), nat.ptr(out), nat.ptr(self._bsum), nat.stream_ptr()), "scan")
         return out
 
-    def _transpose(self, P: DevCSR) -> DevCSR:
+    def _transpose(self, P: DevCSR, with_perm: bool = False):
+        """R = P^T (columns sorted in each row). With ``with_perm`` also the entry map perm with
+        R.val[j] = P.val[perm[j]] (what a numeric refresh re-gathers R through): the transpose
+        is then made of entry numbers once and R's values gathered through it."""
         L, st, dev = nat.lib(), nat.stream_ptr(), self.device
         na = P.ncols
         cnt = torch.zeros(na, dtype=torch.int32, device=dev)
@@ -178,9 +183,18 @@ class SmoothedAggregationAMG:
         fill = torch.zeros(na, dtype=torch.int32, device=dev)
         rcol = torch.empty(P.nnz, dtype=torch.int32, device=dev)
         rval = torch.empty(P.nnz, dtype=torch.float64, device=dev)
-        nat.check(L.ftmi_amgs_transpose_fill(P.n, na, nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(P.val), nat.ptr(rptr),
-                                             nat.ptr(fill), nat.ptr(rcol), nat.ptr(rval), st), "transpose")
-        return DevCSR(na, P.n, rptr, rcol, rval)
+        if not with_perm:
+            nat.check(L.ftmi_amgs_transpose_fill(P.n, na, nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(P.val),
+                                                 nat.ptr(rptr), nat.ptr(fill), nat.ptr(rcol), nat.ptr(rval), st),
+                      "transpose")
+            return DevCSR(na, P.n, rptr, rcol, rval)
+        idx = torch.arange(P.nnz, dtype=torch.float64, device=dev)
+        rnum = torch.empty(P.nnz, dtype=torch.float64, device=dev)
+        nat.check(L.ftmi_amgs_transpose_fill(P.n, na, nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(idx), nat.ptr(rptr),
+                                             nat.ptr(fill), nat.ptr(rcol), nat.ptr(rnum), st), "transpose perm")
+        perm = rnum.to(torch.int64)
+        nat.check(L.ftmi_gather_f64(P.nnz, nat.ptr(perm), nat.ptr(P.val), nat.ptr(rval), st), "gather")
+        return DevCSR(na, P.n, rptr, rcol, rval), perm
 
     def _setup(self, A: DevCSR, host_pattern):
         import time
@@ -232,15 +246,26 @@ class SmoothedAggregationAMG:
             n = A.n
             diag, dinv, mask = first if first is not None else self._strength_mask(A)
             first = None
-            # the strength mask (and a coarse level's pattern) go to pinned host memory while
-            # the GPU runs the power iteration, which does not depend on the aggregates
-            mask_p = _pinned(A.nnz, torch.uint8)
-            mask_p.copy_(mask, non_blocking=True)
-            if pattern is None:
-                ptr_p, col_p = _pinned(n + 1, torch.int32), _pinned(A.nnz, torch.int32)
-                ptr_p.copy_(A.ptr, non_blocking=True)
-                col_p.copy_(A.col, non_blocking=True)
-                pattern = (ptr_p.numpy(), col_p.numpy())
+            # The strength graph goes to pinned host memory while the GPU runs the power
+            # iteration, which does not depend on the aggregates: the mask when the host holds
+            # the level's pattern (the finest level), else the strong entries alone as a CSR
+            # of their own (a coarse level keeps a fraction of its entries).
+            if pattern is not None:
+                mask_p = _pinned(A.nnz, torch.uint8)
+                mask_p.copy_(mask, non_blocking=True)
+                strong = None
+            else:
+                cnt = torch.empty(n, dtype=torch.int32, device=dev)
+                nat.check(L.ftmi_amgs_strong_count(n, nat.ptr(A.ptr), nat.ptr(mask), nat.ptr(cnt), st), "strong count")
+                s_ptr = self._scan(cnt)
+                s_nnz = int(s_ptr[n].item())
+                s_col = torch.empty(max(s_nnz, 1), dtype=torch.int32, device=dev)
+                nat.check(L.ftmi_amgs_strong_fill(n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(mask), nat.ptr(s_ptr),
+                                                  nat.ptr(s_col), st), "strong fill")
+                sp_p, sc_p = _pinned(n + 1, torch.int32), _pinned(max(s_nnz, 1), torch.int32)
+                sp_p.copy_(s_ptr, non_blocking=True)
+                sc_p.copy_(s_col, non_blocking=True)
+                strong = (sp_p.numpy(), sc_p.numpy())
             copied = torch.cuda.Event()
             copied.record()
             x = start_vector_device(n, dev)
@@ -250,12 +275,16 @@ class SmoothedAggregationAMG:
             t0 = clock()
             copied.synchronize()
             t1 = clock()
-            ptr_h, col_h = pattern
-            mask_h = mask_p.numpy()
             agg_p, size_p = _pinned(n, torch.int32), _pinned(n, torch.int32)
             agg_h, size_h = agg_p.numpy(), size_p.numpy()
-            na = int(L.ftmi_amg_aggregate_mask_host(n, ptr_h.ctypes.data, col_h.ctypes.data, mask_h.ctypes.data,
-                                                    agg_h.ctypes.data, size_h.ctypes.data))
+            if strong is None:
+                ptr_h, col_h = pattern
+                mask_h = mask_p.numpy()
+                na = int(L.ftmi_amg_aggregate_mask_host(n, ptr_h.ctypes.data, col_h.ctypes.data, mask_h.ctypes.data,
+                                                        agg_h.ctypes.data, size_h.ctypes.data))
+            else:
+                na = int(L.ftmi_amg_aggregate_csr_host(n, strong[0].ctypes.data, strong[1].ctypes.data,
+                                                       agg_h.ctypes.data, size_h.ctypes.data))
             self.timing["wait_mask"] += t1 - t0
             self.timing["aggregate"] += clock() - t1
             if na >= n * 0.9:
@@ -272,8 +301,10 @@ class SmoothedAggregationAMG:
             plan = {"T": T, "diag": diag, "y": y, "rho": rho, "kP": {}, "kAP": {}, "kRAP": {}}
             P = self._spgemm(A, T, xscale=dinv, agg=agg, tval=t_val, omega_num=rho, omega_scale=4.0 / 3.0,
                              keep=plan["kP"])
-            R = self._transpose(P)
-            plan["perm"] = self._transpose_perm(P, R) if self.refreshable else None
+            if self.refreshable:
+                R, plan["perm"] = self._transpose(P, with_perm=True)
+            else:
+                R, plan["perm"] = self._transpose(P), None
             AP = self._spgemm(A, P, keep=plan["kAP"])
             Ac = self._spgemm(R, AP, keep=plan["kRAP"])
             plan["AP"] = AP
@@ -353,18 +384,6 @@ class SmoothedAggregationAMG:
         except np.linalg.LinAlgError:
             return np.linalg.pinv(Ad)
 
-    def _transpose_perm(self, P: DevCSR, R: DevCSR) -> torch.Tensor:
-        """perm with R.val[j] = P.val[perm[j]]: the transpose applied to the entry numbers."""
-        L, dev = nat.lib(), self.device
-        idx = torch.arange(P.nnz, dtype=torch.float64, device=dev)
-        fill = torch.zeros(P.ncols, dtype=torch.int32, device=dev)
-        rcol = torch.empty(P.nnz, dtype=torch.int32, device=dev)
-        rnum = torch.empty(P.nnz, dtype=torch.float64, device=dev)
-        nat.check(L.ftmi_amgs_transpose_fill(P.n, P.ncols, nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(idx),
-                                             nat.ptr(R.ptr), nat.ptr(fill), nat.ptr(rcol), nat.ptr(rnum),
-                                             nat.stream_ptr()), "transpose perm")
-        return rnum.to(torch.int64)
-
     # ------------------------------------------------------------------ numeric refresh
     def refresh(self):
         """Recompute every value of the hierarchy for new level-0 values (already in
@@ -419,9 +438,10 @@ class SmoothedAggregationAMG:
         self.handle = L.ftmi_amg_create(float(self.omega), int(self.sweeps))
         self._vecs = []
         for A, P, R, dinv in self.levels:
-            v = [torch.zeros(A.n, dtype=torch.float64, device=dev) for _ in range(4)]
+            # the V-cycle writes each level vector before reading it
+            v = [torch.empty(A.n, dtype=torch.float64, device=dev) for _ in range(4)]
             self._vecs.append(v)
-            nat.check(L.ftmi_amg_add_level(self.handle, A.n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val),
+            nat.check(L.ftmi_amg_add_level(self.handle, A.n, A.nnz, P.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val),
                                            nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(P.val), nat.ptr(R.ptr),
                                            nat.ptr(R.col), nat.ptr(R.val), nat.ptr(dinv), *[nat.ptr(t) for t in v]),
                       "amg_add_level")
@@ -482,9 +502,9 @@ class AmgFP32:
         for A, P, R, dinv in src.levels:
             av, pv, rv, dv = to_f32(A.val), to_f32(P.val), to_f32(R.val), to_f32(dinv)
             self._vals.append((av, pv, rv, dv))
-            v = [torch.zeros(A.n, dtype=torch.float32, device=dev) for _ in range(4)]
+            v = [torch.empty(A.n, dtype=torch.float32, device=dev) for _ in range(4)]
             self._keep += [av, pv, rv, dv] + v
-            nat.check(L.ftmi_amg_add_level_f32(self.handle, A.n, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(av),
+            nat.check(L.ftmi_amg_add_level_f32(self.handle, A.n, A.nnz, P.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(av),
                                                nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(pv), nat.ptr(R.ptr),
                                                nat.ptr(R.col), nat.ptr(rv), nat.ptr(dv), *[nat.ptr(t) for t in v]),
                       "amg_add_level_f32")

@@ -117,6 +117,33 @@ __global__ void __launch_bounds__(kBlock) k_strength(int64_t n, const int32_t* _
     }
 }
 
+// the strength graph as its own CSR (the strong entries of each row in column order): row
+// counts, then (after a scan of the counts) the columns
+__global__ void __launch_bounds__(kBlock) k_strong_count(int64_t n, const int32_t* __restrict__ ptr,
+                                                         const uint8_t* __restrict__ mask, int32_t* __restrict__ cnt) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        int32_t c = 0;
+        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j) c += mask[j];
+        cnt[i] = c;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_strong_fill(int64_t n, const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const int32_t* __restrict__ s_ptr,
+                                                        int32_t* __restrict__ s_col) {
+    int64_t lo, hi;
+    block_range(n, lo, hi);
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+        int32_t q = s_ptr[i];
+        for (int32_t j = ptr[i]; j < ptr[i + 1]; ++j)
+            if (mask[j]) s_col[q++] = col[j];
+    }
+}
+
 // T as a CSR matrix with one entry per row: (agg[i], 1/sqrt(size of aggregate agg[i]))
 __global__ void __launch_bounds__(kBlock) k_tentative(int64_t n, const int32_t* __restrict__ agg,
                                                       const int32_t* __restrict__ agg_size,
@@ -712,6 +739,19 @@ int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t
     const int tpr = rows_tpr(n, nnz);
     ROWS_TPR_LAUNCH(tpr, k_strength, rows_grid(n, tpr), s, n, ptr, col, val, diag, theta, mask);
     FTMI_LAUNCH_CHECK("ftmi_amgs_strength");
+    return 0;
+}
+
+int ftmi_amgs_strong_count(int64_t n, const int32_t* ptr, const uint8_t* mask, int32_t* cnt, void* stream) {
+    if (n > 0) k_strong_count<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, ptr, mask, cnt);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_strong_count");
+    return 0;
+}
+
+int ftmi_amgs_strong_fill(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
+                          const int32_t* s_ptr, int32_t* s_col, void* stream) {
+    if (n > 0) k_strong_fill<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, ptr, col, mask, s_ptr, s_col);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_strong_fill");
     return 0;
 }
 

@@ -74,13 +74,18 @@ int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* id
 }
 
 // The same aggregation over a CSR pattern and a strength mask (mask[j] != 0: entry j is a
-// strong off-diagonal connection), int32 in and out; size[a] receives each aggregate's size.
-// Pass 2 (join the first aggregated strong neighbour, in order), the collection of the nodes
-// left for pass 3 and the aggregate sizes share one scan. (Pass 1, the greedy choice of roots,
-// is inherently ordered: a block-parallel replay does not re-synchronise with the sequential
-// tiling on structured orderings, so it stays one thread.)
-int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
-                                     int32_t* agg, int32_t* size) {
+// strong off-diagonal connection; MASK = false: every entry is strong, the graph given as its
+// own CSR), int32 in and out; size[a] receives each aggregate's size. Pass 2 (join the first
+// aggregated strong neighbour, in order), the collection of the nodes left for pass 3 and the
+// aggregate sizes share one scan. (Pass 1, the greedy choice of roots, is inherently ordered:
+// a change of one strength entry can move the roots of a large part of the rows after it, so
+// neither a block-parallel replay nor an incremental one re-synchronises with the sequential
+// result; it stays one thread.)
+}  // extern "C"
+
+template <bool MASK>
+static int64_t aggregate_rows(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask, int32_t* agg,
+                              int32_t* size) {
     if (n <= 0) return 0;
     std::memset(agg, 0xff, (size_t)n * sizeof(int32_t));  // -1
     int32_t na = 0;
@@ -89,7 +94,7 @@ int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_
         const int32_t e = ptr[i + 1];
         bool all_free = true;
         for (int32_t k = ptr[i]; k < e; ++k) {
-            if (mask[k] && agg[col[k]] >= 0) {
+            if ((!MASK || mask[k]) && agg[col[k]] >= 0) {
                 all_free = false;
                 break;
             }
@@ -97,7 +102,7 @@ int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_
         if (!all_free) continue;
         agg[i] = na;
         for (int32_t k = ptr[i]; k < e; ++k)
-            if (mask[k]) agg[col[k]] = na;
+            if (!MASK || mask[k]) agg[col[k]] = na;
         size[na] = 0;
         ++na;
     }
@@ -106,7 +111,7 @@ int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_
         int32_t a = agg[i];
         if (a < 0) {
             for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-                if (mask[k] && agg[col[k]] >= 0) {
+                if ((!MASK || mask[k]) && agg[col[k]] >= 0) {
                     a = agg[col[k]];
                     break;
                 }
@@ -125,6 +130,18 @@ int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_
         ++na;
     }
     return na;
+}
+
+extern "C" {
+
+int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
+                                     int32_t* agg, int32_t* size) {
+    return aggregate_rows<true>(n, ptr, col, mask, agg, size);
+}
+
+int64_t ftmi_amg_aggregate_csr_host(int64_t n, const int32_t* s_ptr, const int32_t* s_col, int32_t* agg,
+                                    int32_t* size) {
+    return aggregate_rows<false>(n, s_ptr, s_col, nullptr, agg, size);
 }
 
 

@@ -1030,7 +1030,8 @@ Amg<T>* amg_of(void* h, const char* where) {
 }
 
 template <typename T>
-int amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const T* a_val,
+int amg_add_level(void* h, int64_t n, int64_t nnz_a, int64_t nnz_p, const int32_t* a_ptr, const int32_t* a_col,
+                  const T* a_val,
                   const int32_t* p_ptr, const int32_t* p_col, const T* p_val, const int32_t* r_ptr,
                   const int32_t* r_col, const T* r_val, const T* dinv, T* vb, T* vx, T* vy, T* vr) {
     Amg<T>* M = amg_of<T>(h, "ftmi_amg_add_level: handle precision does not match the values");
@@ -1044,13 +1045,6 @@ int amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col
     L.A = CsrOp<T>{a_ptr, a_col, a_val};
     L.P = CsrOp<T>{p_ptr, p_col, p_val};
     L.R = CsrOp<T>{r_ptr, r_col, r_val};
-    // row lengths from the CSR pointers (host copies of the last pointer entries)
-    int32_t nnz_a = 0, nnz_p = 0;
-    if (hipMemcpy(&nnz_a, a_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(&nnz_p, p_ptr + n, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) {
-        set_error_msg("ftmi_amg_add_level: cannot read row pointers");
-        return -1;
-    }
     L.tpr_a = pick_tpr(n, nnz_a);
     L.tpr_p = pick_tpr(n, nnz_p);
     L.nnz_a = nnz_a;
@@ -1159,19 +1153,19 @@ void ftmi_amg_destroy(void* h) {
         delete (Amg<double>*)b;
 }
 
-int ftmi_amg_add_level(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const double* a_val,
-                       const int32_t* p_ptr, const int32_t* p_col, const double* p_val, const int32_t* r_ptr,
-                       const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
-                       double* vy, double* vr) {
-    return amg_add_level<double>(h, n, a_ptr, a_col, a_val, p_ptr, p_col, p_val, r_ptr, r_col, r_val, dinv, vb, vx,
+int ftmi_amg_add_level(void* h, int64_t n, int64_t nnz_a, int64_t nnz_p, const int32_t* a_ptr, const int32_t* a_col,
+                       const double* a_val, const int32_t* p_ptr, const int32_t* p_col, const double* p_val,
+                       const int32_t* r_ptr, const int32_t* r_col, const double* r_val, const double* dinv, double* vb,
+                       double* vx, double* vy, double* vr) {
+    return amg_add_level<double>(h, n, nnz_a, nnz_p, a_ptr, a_col, a_val, p_ptr, p_col, p_val, r_ptr, r_col, r_val, dinv, vb, vx,
                                  vy, vr);
 }
 
-int ftmi_amg_add_level_f32(void* h, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const float* a_val,
-                           const int32_t* p_ptr, const int32_t* p_col, const float* p_val, const int32_t* r_ptr,
-                           const int32_t* r_col, const float* r_val, const float* dinv, float* vb, float* vx,
-                           float* vy, float* vr) {
-    return amg_add_level<float>(h, n, a_ptr, a_col, a_val, p_ptr, p_col, p_val, r_ptr, r_col, r_val, dinv, vb, vx,
+int ftmi_amg_add_level_f32(void* h, int64_t n, int64_t nnz_a, int64_t nnz_p, const int32_t* a_ptr,
+                           const int32_t* a_col, const float* a_val, const int32_t* p_ptr, const int32_t* p_col,
+                           const float* p_val, const int32_t* r_ptr, const int32_t* r_col, const float* r_val,
+                           const float* dinv, float* vb, float* vx, float* vy, float* vr) {
+    return amg_add_level<float>(h, n, nnz_a, nnz_p, a_ptr, a_col, a_val, p_ptr, p_col, p_val, r_ptr, r_col, r_val, dinv, vb, vx,
                                 vy, vr);
 }
 

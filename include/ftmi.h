@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 104
+#define FTMI_ABI_VERSION 105
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -193,7 +193,10 @@ void ftmi_amg_destroy(void* amg);
 int ftmi_set_tpr_bias(int bias);
 /* new: re-pick the threads per row of every level of hierarchy h with the current shift */
 int ftmi_amg_retune(void* h);
-int ftmi_amg_add_level(void* amg, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const double* a_val,
+/* nnz_a, nnz_p: entries of A and P (the last row pointers), given so that adding a level
+ * reads nothing back from the device */
+int ftmi_amg_add_level(void* amg, int64_t n, int64_t nnz_a, int64_t nnz_p, const int32_t* a_ptr, const int32_t* a_col,
+                       const double* a_val,
                        const int32_t* p_ptr, const int32_t* p_col, const double* p_val, const int32_t* r_ptr,
                        const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
                        double* vy, double* vr);
@@ -202,7 +205,8 @@ int ftmi_amg_vcycle(void* amg, const double* b, double* x, void* stream);
 /* the same hierarchy with FP32 values and vectors (new: the mixed-precision pressure solve);
  * a handle takes only the entry points of its own precision */
 void* ftmi_amg_create_f32(double omega, int sweeps);
-int ftmi_amg_add_level_f32(void* amg, int64_t n, const int32_t* a_ptr, const int32_t* a_col, const float* a_val,
+int ftmi_amg_add_level_f32(void* amg, int64_t n, int64_t nnz_a, int64_t nnz_p, const int32_t* a_ptr,
+                           const int32_t* a_col, const float* a_val,
                            const int32_t* p_ptr, const int32_t* p_col, const float* p_val, const int32_t* r_ptr,
                            const int32_t* r_col, const float* r_val, const float* dinv, float* vb, float* vx,
                            float* vy, float* vr);
@@ -215,6 +219,10 @@ int64_t ftmi_amg_aggregate_host(int64_t n, const int64_t* ptr, const int64_t* id
  * (int32 in/out); size[a] = members of aggregate a (size holds n entries) */
 int64_t ftmi_amg_aggregate_mask_host(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
                                      int32_t* agg, int32_t* size);
+/* new: the same aggregation over the strength graph as its own CSR (s_ptr, s_col: the strong
+ * entries of each row in column order, from ftmi_amgs_strong_count / _fill) */
+int64_t ftmi_amg_aggregate_csr_host(int64_t n, const int32_t* s_ptr, const int32_t* s_col, int32_t* agg,
+                                    int32_t* size);
 
 /* new: device reductions of the outer-iteration diagnostics (flowtrace/unstructured/diagnostics.py:
  * 103-181 _rows_of_system / _row_set, 214-290 evaluate); results land in device doubles.
@@ -262,6 +270,11 @@ int ftmi_amgs_diag(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* co
 /* strength mask |a_ij| >= theta sqrt(|a_ii a_jj|), i != j (amg.py:60-66) */
 int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
                        const double* diag, double theta, uint8_t* mask, void* stream);
+/* new: the strength graph of a level as its own CSR: per-row counts of the mask's strong
+ * entries, then (after ftmi_scan_i32 of the counts into s_ptr) their columns in order */
+int ftmi_amgs_strong_count(int64_t n, const int32_t* ptr, const uint8_t* mask, int32_t* cnt, void* stream);
+int ftmi_amgs_strong_fill(int64_t n, const int32_t* ptr, const int32_t* col, const uint8_t* mask,
+                          const int32_t* s_ptr, int32_t* s_col, void* stream);
 /* tentative prolongator as CSR: t_ptr = 0..n, t_val = 1/sqrt(|aggregate|) */
 int ftmi_amgs_tentative(int64_t n, const int32_t* agg, const int32_t* agg_size, int32_t* t_ptr, double* t_val,
                         void* stream);

@@ -84,6 +84,26 @@ def test_host_aggregation_matches_oracle():
     assert np.array_equal(agg, g["L0_agg"]) and na == int(g["L0_na"])
     agg2, na2 = O.aggregate(S.indptr, S.indices, S.shape[0])
     assert np.array_equal(agg, agg2) and na == na2
+    # the forms the device set-up hands over: the pattern with a strength mask per entry (the
+    # finest level) and the strong entries as a CSR of their own (coarse levels)
+    from gputest_amd import _native as nat
+
+    A.sort_indices()
+    n = A.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(A.indptr))
+    dg = np.abs(A.diagonal())
+    mask = ((A.indices != rows) & (np.abs(A.data) >= 0.08 * np.sqrt(dg[rows] * dg[A.indices]))).astype(np.uint8)
+    ptr, col = np.ascontiguousarray(A.indptr, np.int32), np.ascontiguousarray(A.indices, np.int32)
+    sptr, scol = np.ascontiguousarray(S.indptr, np.int32), np.ascontiguousarray(S.indices, np.int32)
+    L = nat.lib()
+    for call in (lambda a, z: L.ftmi_amg_aggregate_mask_host(n, ptr.ctypes.data, col.ctypes.data, mask.ctypes.data,
+                                                            a.ctypes.data, z.ctypes.data),
+                 lambda a, z: L.ftmi_amg_aggregate_csr_host(n, sptr.ctypes.data, scol.ctypes.data, a.ctypes.data,
+                                                           z.ctypes.data)):
+        a32, size = np.empty(n, np.int32), np.empty(n, np.int32)
+        assert call(a32, size) == na
+        assert np.array_equal(a32, agg)
+        assert np.array_equal(size[:na], np.bincount(agg, minlength=na))
 
 
 def test_assemble_plan_covers_every_slot():
