@@ -813,12 +813,56 @@ __device__ inline T row_dot(const CsrOp<T>& A, int64_t i, const T* __restrict__ 
 
 inline int grid_rows(int64_t n, int tpr) { return grid_for(n * tpr); }
 
+// The end of PCG iteration k, folded into the first kernel of its preconditioner: ||r|| from
+// the partials of the x/r update, the convergence test and the host post (the work of a
+// one-workgroup finalize kernel). `part` == nullptr: nothing to finish.
+struct PcgFinish {
+    State* S;
+    const double* part;
+    int np;
+    double tol;
+    const double* tol_dev;
+    int k;
+    Post post;
+};
+
+// true when the solve is done (then the caller returns); every thread gets the same answer
+__device__ inline bool pcg_finish(const PcgFinish& f) {
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    State* S = f.S;
+    if (S->st[I_DONE]) {
+        if (lead) post_host(S, f.post);
+        return true;
+    }
+    double tot[1];
+    consume_sum<1>(f.part, f.np, tot);
+    const double tol = f.tol_dev ? *f.tol_dev : f.tol;
+    const double rn = sqrt(tot[0]);
+    const double res = rn / S->sc[S_BNORM];
+    const bool done = res < tol;
+    if (lead) {
+        S->sc[S_RNORM] = rn;
+        S->sc[S_RES] = res;
+        if (done) {
+            S->st[I_DONE] = 1;
+            S->st[I_CONV] = 1;
+            S->st[I_ITERS] = f.k;
+        }
+        post_host(S, f.post);
+    }
+    return done;
+}
+
 // x = w D b + w D (b - A (w D b)): the first two pre-smoothing sweeps from x = 0, fused.
 template <int TPR, typename T>
 __global__ void __launch_bounds__(kBlock) vc_pre2(int64_t n, CsrOp<T> A, const T* __restrict__ Dinv,
                                                   const T* __restrict__ b, T* __restrict__ x, double w_d,
-                                                  const int* done) {
-    if (done && *done) return;
+                                                  const int* done, PcgFinish fin) {
+    if (fin.part) {
+        if (pcg_finish(fin)) return;
+    } else if (done && *done) {
+        return;
+    }
     const T w = (T)w_d;
     RowGroups<TPR> G;
     for (int64_t i = G.first; i < n; i += G.stride) {
@@ -842,8 +886,10 @@ __global__ void __launch_bounds__(kBlock) vc_pre1(int64_t n, const T* __restrict
     for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) x[i] = (w * Dinv[i]) * b[i];
 }
 
-// y = x + w D (b - A x)   (optionally also the workgroup partial sums of y)
-template <int TPR, bool SUM, typename T>
+// y = x + w D (b - A x), optionally with the workgroup partial sums of y (RED = kRedSum) or
+// of b . y (RED = kRedDot: with b the PCG residual r and y = M r, the r . z of the iteration)
+enum { kRedNone = 0, kRedSum = 1, kRedDot = 2 };
+template <int TPR, int RED, typename T>
 __global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<T> A, const T* __restrict__ Dinv,
                                                     const T* __restrict__ b, const T* __restrict__ x,
                                                     T* __restrict__ y, double w_d, const int* done,
@@ -855,12 +901,14 @@ __global__ void __launch_bounds__(kBlock) vc_jacobi(int64_t n, CsrOp<T> A, const
     for (int64_t i = G.first; i < n; i += G.stride) {
         const T ax = row_dot<TPR>(A, i, x, G.lane);
         if (G.lane == 0) {
-            const T yi = x[i] + (w * Dinv[i]) * (b[i] - ax);
+            const T bi = b[i];
+            const T yi = x[i] + (w * Dinv[i]) * (bi - ax);
             y[i] = yi;
-            if (SUM) acc[0] += yi;
+            if (RED == kRedSum) acc[0] += yi;
+            if (RED == kRedDot) acc[0] += bi * yi;
         }
     }
-    if (SUM) block_partials<1>(acc, sum_partials);
+    if (RED != kRedNone) block_partials<1>(acc, sum_partials);
 }
 
 template <int TPR, typename T>
@@ -949,28 +997,35 @@ inline int sum_grid(int64_t n, int tpr) {
 
 template <int TPR, typename T>
 static void launch_jacobi(int64_t n, CsrOp<T> A, const T* Dinv, const T* b, const T* x, T* y, double w,
-                          const int* done, double* zsum, hipStream_t st) {
-    if (zsum)
-        vc_jacobi<TPR, true, T><<<sum_grid(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, zsum);
+                          const int* done, double* part, int red, hipStream_t st) {
+    if (part && red == kRedSum)
+        vc_jacobi<TPR, kRedSum, T><<<sum_grid(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, part);
+    else if (part && red == kRedDot)
+        vc_jacobi<TPR, kRedDot, T><<<sum_grid(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, part);
     else
-        vc_jacobi<TPR, false, T><<<grid_rows(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, nullptr);
+        vc_jacobi<TPR, kRedNone, T><<<grid_rows(n, TPR), kBlock, 0, st>>>(n, A, Dinv, b, x, y, w, done, nullptr);
 }
 
 template <typename T>
 static void jacobi(int tpr, int64_t n, CsrOp<T> A, const T* Dinv, const T* b, const T* x, T* y, double w,
-                   const int* done, double* zsum, hipStream_t st) {
+                   const int* done, double* part, int red, hipStream_t st) {
     switch (tpr) {
-        case 1: launch_jacobi<1, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
-        case 4: launch_jacobi<4, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
-        case 16: launch_jacobi<16, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
-        default: launch_jacobi<64, T>(n, A, Dinv, b, x, y, w, done, zsum, st); break;
+        case 1: launch_jacobi<1, T>(n, A, Dinv, b, x, y, w, done, part, red, st); break;
+        case 4: launch_jacobi<4, T>(n, A, Dinv, b, x, y, w, done, part, red, st); break;
+        case 16: launch_jacobi<16, T>(n, A, Dinv, b, x, y, w, done, part, red, st); break;
+        default: launch_jacobi<64, T>(n, A, Dinv, b, x, y, w, done, part, red, st); break;
     }
 }
 
-// Enqueue one V-cycle on level k: x_k = M_k^{-1} b_k. When `zsum` is given, the last
-// smoothing sweep also writes the workgroup partial sums of the output (sum_grid of them).
+constexpr PcgFinish kNoFinish{nullptr, nullptr, 0, 0.0, nullptr, 0, kNoPost};
+
+// Enqueue one V-cycle on level k: x_k = M_k^{-1} b_k. With `part`, the last smoothing sweep
+// also writes workgroup partial sums (sum_grid of them): of the output (red = kRedSum) or of
+// b_k . x_k (red = kRedDot). With fin.part set, the level's first kernel first finishes the PCG
+// iteration that produced b_k (pcg_finish); it needs at least two sweeps.
 template <typename T>
-static void vcycle(const Amg<T>& M, int k, hipStream_t st, const int* done, double* zsum) {
+static void vcycle(const Amg<T>& M, int k, hipStream_t st, const int* done, double* part, int red = kRedSum,
+                   PcgFinish fin = kNoFinish) {
     if (k == M.nlev) {
         const int rows_per_block = kBlock / kWave;
         vc_coarse<T><<<(int)((M.nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M.nc, M.cinv, M.cb,
@@ -986,9 +1041,9 @@ static void vcycle(const Amg<T>& M, int k, hipStream_t st, const int* done, doub
     T* xa = L.x;
     T* xb = L.y;
     if (M.sweeps >= 2) {
-        TPR_LAUNCH(L.tpr_a, vc_pre2, L.n, st, L.n, L.A, L.Dinv, L.b, xa, w, done);
+        TPR_LAUNCH(L.tpr_a, vc_pre2, L.n, st, L.n, L.A, L.Dinv, L.b, xa, w, done, fin);
         for (int s = 2; s < M.sweeps; ++s) {
-            jacobi<T>(L.tpr_a, L.n, L.A, L.Dinv, L.b, xa, xb, w, done, nullptr, st);
+            jacobi<T>(L.tpr_a, L.n, L.A, L.Dinv, L.b, xa, xb, w, done, nullptr, kRedNone, st);
             T* t = xa;
             xa = xb;
             xb = t;
@@ -1005,7 +1060,7 @@ static void vcycle(const Amg<T>& M, int k, hipStream_t st, const int* done, doub
     T* other = (xa == L.x) ? L.y : L.x;
     for (int s = 0; s < M.sweeps; ++s) {
         const bool last = (s == M.sweeps - 1);
-        jacobi<T>(L.tpr_a, L.n, L.A, L.Dinv, L.b, cur, other, w, done, last ? zsum : nullptr, st);
+        jacobi<T>(L.tpr_a, L.n, L.A, L.Dinv, L.b, cur, other, w, done, last ? part : nullptr, red, st);
         T* t = cur;
         cur = other;
         other = t;
@@ -1416,8 +1471,28 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
         pcg_init_b<false, T, XT><<<g, kBlock, 0, st>>>(n, b, W.r, x, zx, W.S, P0, g, P1);
     }
     fin_pcg_init<<<1, kBlock, 0, st>>>(P1, g, W.S, tol, g_capture ? kNoPost : next_post(pn, 0), tol_dev);
-    // z = M r, with the partial sums of z in P2 when projecting; returns their count
-    auto precond = [&]() -> int {
+    // z = M r and the partials of r . z in P0; returns their count. With a V-cycle of two or
+    // more sweeps, `fin` (the end of the previous iteration: ||r||, convergence, host post) is
+    // done by the cycle's first kernel and r . z by its last sweep (with the projection, the
+    // last sweep sums z and pcg_z removes the mean and forms r . z); otherwise `fin` is a
+    // finalize kernel of its own and pcg_z forms r . z.
+    const bool fold = M && M->nlev > 0 && M->sweeps >= 2;
+    auto precond = [&](PcgFinish fin) -> int {
+        if (fold) {
+            Level<T>& L0 = M->lv[0];
+            T *sb = L0.b, *sx = L0.x;
+            L0.b = W.r;
+            L0.x = W.z;
+            const int nz = sum_grid(n, L0.tpr_a);
+            vcycle<T>(*M, 0, st, done, project ? P2 : P0, project ? kRedSum : kRedDot, fin);
+            L0.b = sb;
+            L0.x = sx;
+            if (!project) return nz;
+            pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+            return g;
+        }
+        if (fin.part) fin_pcg_xr<<<1, kBlock, 0, st>>>(fin.part, fin.np, W.S, fin.tol, fin.k, fin.post, fin.tol_dev);
+        int nz = g;
         if (M && M->nlev > 0) {
             Level<T>& L0 = M->lv[0];
             T *sb = L0.b, *sx = L0.x;
@@ -1426,35 +1501,31 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
             vcycle<T>(*M, 0, st, done, project ? P2 : nullptr);
             L0.b = sb;
             L0.x = sx;
-            return sum_grid(n, L0.tpr_a);
-        }
-        if (M) {
-            const int rpb = kBlock / kWave;
-            vc_coarse<T><<<(int)((M->nc + rpb - 1) / rpb), kBlock, 0, st>>>(M->nc, M->cinv, W.r, W.z, done);
+            nz = sum_grid(n, L0.tpr_a);
         } else {
-            vc_copy<T><<<grid_for(n), kBlock, 0, st>>>(n, W.r, W.z, done);
+            if (M) {
+                const int rpb = kBlock / kWave;
+                vc_coarse<T><<<(int)((M->nc + rpb - 1) / rpb), kBlock, 0, st>>>(M->nc, M->cinv, W.r, W.z, done);
+            } else {
+                vc_copy<T><<<grid_for(n), kBlock, 0, st>>>(n, W.r, W.z, done);
+            }
+            if (project) k_sum<T><<<g, kBlock, 0, st>>>(n, W.z, done, P2);
         }
-        if (project) k_sum<T><<<g, kBlock, 0, st>>>(n, W.z, done, P2);
+        if (project)
+            pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+        else
+            pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
         return g;
     };
-    int nz = precond();
-    if (project)
-        pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
-    else
-        pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+    int nrz = precond(kNoFinish);
     FTMI_LAUNCH_CHECK("pcg init");
     if (g_capture) {
         if (cap < 0) report = capture_slot(maxiter, &cap);
         for (int64_t k = 1; k <= cap; ++k) {
             const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-            pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
+            pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, nrz, P1, k == 1, (int)k);
             pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
-            fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, kNoPost, tol_dev);
-            nz = precond();
-            if (project)
-                pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
-            else
-                pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+            nrz = precond(PcgFinish{W.S, P2, g, tol, tol_dev, (int)k, kNoPost});
         }
         if (report) k_report<<<1, 64, 0, st>>>(W.S, report, cap >= maxiter ? 1 : 0, cap);
         FTMI_LAUNCH_CHECK("pcg (capture)");
@@ -1465,14 +1536,9 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-        pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, g, P1, k == 1, (int)k);
+        pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, nrz, P1, k == 1, (int)k);
         pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
-        fin_pcg_xr<<<1, kBlock, 0, st>>>(P2, g, W.S, tol, (int)k, next_post(pn, last ^ 1), tol_dev);
-        nz = precond();
-        if (project)
-            pcg_z<true, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
-        else
-            pcg_z<false, T><<<g, kBlock, 0, st>>>(n, W.r, W.z, W.S, P2, nz, P0);
+        nrz = precond(PcgFinish{W.S, P2, g, tol, tol_dev, (int)k, next_post(pn, last ^ 1)});
         FTMI_LAUNCH_CHECK("pcg iteration");
         if (wait_state(pn, last, st)) return -1;
         const bool prev_done = pn->h[last].st[I_DONE] != 0;
