@@ -83,6 +83,7 @@ _SIGS = {
     "ftmi_bicgstab_jacobi_noalloc": ([I64, P, P, P, P, P, P, F64, I64, I32, P, P, P], I32),
     "ftmi_bicgstab_ldu_sliced": ([I64, P, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
     "ftmi_ldu_slice": ([I64, P, P, P, P], I32),
+    "ftmi_ldu_slice_f32": ([I64, P, P, P, P], I32),
     "ftmi_bicgstab_ldu": ([I64, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
     "ftmi_pcg_work_bytes": ([I64], I64),
     "ftmi_pcg_ldu": ([I64, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
@@ -98,6 +99,7 @@ _SIGS = {
     "ftmi_amg_vcycle_f32": ([P, P, P, P], I32),
     "ftmi_pcg_ldu_f32": ([I64, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
     "ftmi_bicgstab_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, I32, I32, P, P, P, P], I32),
+    "ftmi_bicgstab_ldu_mixed_sliced": ([I64, P, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
     "ftmi_pcg_ldu_mixed": ([I64, P, P, P, P, P, P, P, F64, I64, P, I32, I32, P, P, P, P, P], I32),
     "ftmi_cast_f64_f32": ([I64, P, P, P], I32),
     "ftmi_gather_f64": ([I64, P, P, P, P], I32),

@@ -1754,6 +1754,27 @@ int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, cons
     return conv;
 }
 
+int ftmi_bicgstab_ldu_mixed_sliced(int64_t n, const float* ex, const int32_t* base, const uint8_t* deg,
+                                   const int32_t* nbr, const float* off, const float* b, const float* inv_diag,
+                                   double* x, double tol, int64_t maxiter, int relative_to_initial, void* work,
+                                   int64_t* iters, double* res, void* stream) {
+    if (n <= 0) {
+        *iters = 0;
+        *res = 0.0;
+        return 1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    LduSellDiffOp<float> A{ex, base, deg, nbr, off};
+    float* d = (float*)work + n;
+    void* inner = (char*)work + align256(2 * n * (int64_t)sizeof(float));
+    const int conv = bicgstab_run<LduSellDiffOp<float>, float>(n, A, b, inv_diag, d, tol, maxiter,
+                                                               relative_to_initial, inner, iters, res, st, true);
+    if (conv < 0) return conv;
+    k_add_f32<<<grid_for(n), kBlock, 0, st>>>(n, d, x);
+    FTMI_LAUNCH_CHECK("ftmi_bicgstab_ldu_mixed_sliced update");
+    return conv;
+}
+
 int ftmi_pcg_ldu_mixed(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
                        const float* off32, const double* b, double* x, double tol, int64_t maxiter, void* amg32,
                        int project, int max_refine, void* work, int64_t* iters, double* res, int64_t* rounds,

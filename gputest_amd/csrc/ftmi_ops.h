@@ -122,4 +122,26 @@ struct LduDiffOp {
     }
 };
 
+// The difference-form operator with its half-edges in the 64-row slice layout of LduSellOp:
+// the same row sums in the same order as LduDiffOp, coalesced loads of nbr/off.
+template <typename T>
+struct LduSellDiffOp {
+    const T* ex;
+    const int32_t* base;
+    const uint8_t* deg;
+    const int32_t* nbr;
+    const T* off;
+    __device__ inline T row(int64_t i, const T* __restrict__ x) const {
+        return row_f(i, [&](int32_t j) { return x[j]; });
+    }
+    template <class F>
+    __device__ inline T row_f(int64_t i, F f) const {
+        const T xi = f(i);
+        const int32_t b = base[i >> 6] + (int32_t)(i & 63);
+        return gather_sum(
+            ex[i] * xi, (int)deg[i], (int32_t)i, [&](int k) { return nbr[b + 64 * k]; },
+            [&](int k) { return off[b + 64 * k]; }, [&](int32_t j) { return f(j) - xi; });
+    }
+};
+
 }  // namespace ftmi

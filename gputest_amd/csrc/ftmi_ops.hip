@@ -237,6 +237,12 @@ int ftmi_ldu_slice(int64_t nhe, const int32_t* pos, const double* off, double* s
     return 0;
 }
 
+int ftmi_ldu_slice_f32(int64_t nhe, const int32_t* pos, const float* off, float* soff, void* stream) {
+    if (nhe > 0) k_slice<float><<<grid_for(nhe), kBlock, 0, (hipStream_t)stream>>>(nhe, pos, off, soff);
+    FTMI_LAUNCH_CHECK("ftmi_ldu_slice_f32");
+    return 0;
+}
+
 int ftmi_gather_f64(int64_t n, const int64_t* idx, const double* x, double* y, void* stream) {
     if (n > 0) k_gather_f64<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, idx, x, y);
     FTMI_LAUNCH_CHECK("ftmi_gather_f64");

@@ -323,11 +323,12 @@ class FVMesh:
         self.sl_deg = torch.as_tensor(deg.astype(np.uint8), device=self.device)
 
     def slice_values(self, off, out=None):
-        """Half-edge values (cptr order) in the slice layout (padding entries are never read)."""
+        """Half-edge values (cptr order, FP64 or FP32) in the slice layout (padding entries are
+        never read)."""
         if out is None:
-            out = torch.zeros(self.sl_size, dtype=torch.float64, device=self.device)
-        nat.check(nat.lib().ftmi_ldu_slice(off.numel(), nat.ptr(self.sl_pos), nat.ptr(off), nat.ptr(out),
-                                           nat.stream_ptr()), "ldu_slice")
+            out = torch.zeros(self.sl_size, dtype=off.dtype, device=self.device)
+        fn = nat.lib().ftmi_ldu_slice_f32 if off.dtype == torch.float32 else nat.lib().ftmi_ldu_slice
+        nat.check(fn(off.numel(), nat.ptr(self.sl_pos), nat.ptr(off), nat.ptr(out), nat.stream_ptr()), "ldu_slice")
         return out
 
     # ------------------------------------------------------------------ host helpers

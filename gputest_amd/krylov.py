@@ -110,13 +110,19 @@ def pcg_csr(A, b, tol: float, maxiter: int, amg=None, project=False, ws=None):
 
 
 def bicgstab_ldu_mixed(fv, ex32, off32, b32, inv32, x, tol: float, maxiter: int, relative_to_initial=True,
-                       ws=None, b_is_residual: bool = False):
+                       ws=None, b_is_residual: bool = False, sliced_off=None):
     """FP32 Jacobi-BiCGSTAB for the correction of the FP64 iterate ``x`` (updated in place);
     the operator is (ex32, off32) in difference form, ex32 holding the row sums. ``b32`` is the
     right-hand side, or with ``b_is_residual`` the residual b - A x already formed (the
-    mixed-precision transport assembly writes it that way)."""
+    mixed-precision transport assembly writes it that way). With ``sliced_off`` (``off32`` in
+    ``fv``'s slice layout, ``fv.slice_values``) and ``b_is_residual`` the solve reads the
+    sliced operator: the same sums, coalesced loads."""
     n = b32.shape[0]
     work = (ws or _ws(b32.device)).get("bicgstab", n)
+    if sliced_off is not None and b_is_residual:
+        return _run(nat.lib().ftmi_bicgstab_ldu_mixed_sliced, n, nat.ptr(ex32), nat.ptr(fv.sl_base),
+                    nat.ptr(fv.sl_deg), nat.ptr(fv.sl_nbr), nat.ptr(sliced_off), nat.ptr(b32), nat.ptr(inv32),
+                    nat.ptr(x), float(tol), int(maxiter), int(bool(relative_to_initial)), nat.ptr(work))
     return _run(nat.lib().ftmi_bicgstab_ldu_mixed, n, nat.ptr(ex32), nat.ptr(fv.cptr), nat.ptr(fv.cnbr),
                 nat.ptr(off32), nat.ptr(b32), nat.ptr(inv32), nat.ptr(x), float(tol), int(maxiter),
                 int(bool(relative_to_initial)), int(bool(b_is_residual)), nat.ptr(work))

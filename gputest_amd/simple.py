@@ -216,6 +216,8 @@ class SimpleSolver:
             self.ex32 = f(nc)
             self.b32 = f(dim, nc)
             self.offp32 = f(2 * nf)
+            # the FP32 transport operator in the slice layout, re-sliced after each assembly
+            self.soff32 = f(self.fv.sl_size) if self.fv.sl_base is not None else None
         self.scal = z(_NSCAL + 4 * _MAX_SOLVES)
         self.red_partials = z(2048 * 8)
         self.red_counter = torch.zeros(64, dtype=torch.int32, device=dev)
@@ -442,10 +444,13 @@ class SimpleSolver:
         infos = []
         if self.soff is not None:
             fv.slice_values(self.off, self.soff)
+        if self.mixed and self.soff32 is not None:
+            fv.slice_values(self.off32, self.soff32)
         for k in range(dim):
             if self.mixed:
                 infos.append(bicgstab_ldu_mixed(fv, self.ex32, self.off32, self.b32[k], self.inv_r32,
-                                                self.Ustar[k], 0.1, 30, True, ws=self.ws, b_is_residual=True))
+                                                self.Ustar[k], 0.1, 30, True, ws=self.ws, b_is_residual=True,
+                                                sliced_off=self.soff32))
             else:
                 infos.append(bicgstab_ldu(fv, self.diag_r, self.off, self.bmom[k], self.inv_r, self.Ustar[k], 0.1,
                                           30, True, ws=self.ws, sliced_off=self.soff))
@@ -590,8 +595,10 @@ class SimpleSolver:
             a.extra[0] = nat.ptr(self.en_src)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
         if self.mixed:
+            if self.soff32 is not None:
+                self.fv.slice_values(self.off32, self.soff32)
             info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
-                                      True, ws=self.ws, b_is_residual=True)
+                                      True, ws=self.ws, b_is_residual=True, sliced_off=self.soff32)
         else:
             if self.soff is not None:
                 self.fv.slice_values(self.offT, self.soff)

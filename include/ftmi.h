@@ -129,6 +129,7 @@ int ftmi_bicgstab_ldu_sliced(int64_t n, const double* diag, const int32_t* base,
                              double* res, void* stream);
 /* new: soff[pos[k]] = off[k], half-edge values into the slice layout */
 int ftmi_ldu_slice(int64_t nhe, const int32_t* pos, const double* off, double* soff, void* stream);
+int ftmi_ldu_slice_f32(int64_t nhe, const int32_t* pos, const float* off, float* soff, void* stream);
 int ftmi_bicgstab_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
                       const double* b, const double* inv_diag, double* x, double tol, int64_t maxiter,
                       int relative_to_initial, void* work, int64_t* iters, double* res, void* stream);
@@ -162,6 +163,12 @@ int ftmi_bicgstab_ldu_mixed(int64_t n, const float* ex, const int32_t* ptr, cons
                             const float* b, const float* inv_diag, double* x, double tol, int64_t maxiter,
                             int relative_to_initial, int b_is_residual, void* work, int64_t* iters, double* res,
                             void* stream);
+/* ftmi_bicgstab_ldu_mixed with b_is_residual, the operator's half-edges in the slice layout of
+ * ftmi_bicgstab_ldu_sliced (the same sums, coalesced loads) */
+int ftmi_bicgstab_ldu_mixed_sliced(int64_t n, const float* ex, const int32_t* base, const uint8_t* deg,
+                                   const int32_t* nbr, const float* off, const float* b, const float* inv_diag,
+                                   double* x, double tol, int64_t maxiter, int relative_to_initial, void* work,
+                                   int64_t* iters, double* res, void* stream);
 /* pressure solve by FP64 iterative refinement: r = b - A x in FP64 (projected when
  * `project`), stop when ||r|| / ||b|| < tol, else FP32 AMG-PCG on A32 d = r to the level
  * tol ||b||, its updates added straight into the FP64 x, at most max_refine times (FP32
