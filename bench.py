@@ -284,6 +284,62 @@ def sst_leg(nx, ny, steps, warmup):
             "residuals": s.residuals()["scaled"], "last_linear": s.last_linear, **info}
 
 
+def channel_leg(nx, ny, nz, steps, levels=3, coarse_sweeps=4):
+    """The 3-D channel workload: nx x ny x nz hexes on [0,4]x[0,1]x[0,1], y tanh-stretched
+    (factor 2), periodic in x, driven by the constant pressure gradient of Re_bulk 2000, laminar,
+    from rest at the automatic pressure relaxation (the stability guard on), ``steps`` outer
+    iterations each, FP64 and mixed precision (FP64 continuity residual), both with the pressure
+    preconditioner cut at ``levels`` levels, the coarsest solved by ``coarse_sweeps`` Jacobi
+    sweeps. (The reference's rule, coarsening to 300 unknowns, stops here at a fourth level of
+    8,251 unknowns with 1,390 entries per row, too dense for the device sparse products, and
+    would invert it densely.)"""
+    import torch
+
+    from gputest_amd import case as C
+    from gputest_amd.mesh import channel_mesh
+    from gputest_amd.simple import SimpleSolver
+
+    t0 = time.perf_counter()
+    mesh = channel_mesh(nx, ny, nz)
+    print(f"[bench:channel] mesh {mesh.n_cells} cells in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    mu = 1.0 / 2000.0
+    out = {"workload": f"3-D periodic channel {nx}x{ny}x{nz} hexes ({mesh.n_cells} cells), Re_bulk 2000, laminar",
+           "steps": steps}
+    rates = {}
+    for precision in ("fp64", "mixed"):
+        kw = {"amg_levels": levels, "amg_coarse_sweeps": coarse_sweeps}
+        if precision == "mixed":
+            kw["precision"] = "mixed"
+        case = C.UnstructuredCase(viscosity=mu, periodic_pressure_gradient=[-12.0 * mu, 0.0, 0.0],
+                                  pressure_tol=0.1, max_iter=10 ** 9, tol=0.0, **kw)
+        s = SimpleSolver(mesh, case, "cuda")
+        s.step()  # set-up of the hierarchy and first-use costs out of the timed window
+        g0 = len(s.guard_events)
+        pits = []
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(steps - 1):
+            s.step()
+            pits.append(s.last_linear.get("pressure_iters") or 0)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        rates[precision] = (steps - 1) / dt
+        out[precision] = {"it_per_s": round(rates[precision], 3), "ms_per_step": round(dt / (steps - 1) * 1e3, 3),
+                          "guard_events": len(s.guard_events) - g0, "pressure_iters_total": int(sum(pits)),
+                          "amg_levels": s._amg.n_levels if s._amg is not None else None,
+                          "coarse_unknowns": s._amg.coarse_n if s._amg is not None else None,
+                          "residuals": {k: float(f"{v:.3g}") for k, v in s.residuals()["scaled"].items()}}
+        print(f"[bench:channel] {precision}: {out[precision]}", file=sys.stderr, flush=True)
+        del s
+    out["mixed"]["speedup_vs_fp64"] = round(rates["mixed"] / rates["fp64"], 3)
+    out["note"] = ("from rest the flow's cross-stream mode makes the reference's own FP64 iteration roll back "
+                   "repeatedly (guard events); the converged comparison of mixed against the reference's FP64 "
+                   "solution is tests/test_gpu_mixed.py (16x8x8 with the reference's hierarchy, 32x16x8 with "
+                   "the cut one)")
+    return out
+
+
 def cpu_baseline(nx, ny, warmup, iters):
     """The reference's CPU path on the same workload: the oracle's restatement of the NumPy host
     code with the reference's own C++/OpenMP kernels (``oracle/native_cpu.py``, compiled from
@@ -348,7 +404,11 @@ def gpu_vs_oracle(nx, ny, fields) -> dict:
 
     return {"iterations": fields["iterations"], "U": float(f"{rel(U, fields['U']):.3g}"),
             "p": float(f"{rel(p - p.mean(), fields['p'] - fields['p'].mean()):.3g}"),
-            "T": float(f"{rel(T - 300.0, fields['T'] - 300.0):.3g}")}
+            "T": float(f"{rel(T - 300.0, fields['T'] - 300.0):.3g}"),
+            "note": "mid-run T is not iterate-stable: its 60-step Jacobi-BiCGSTAB stops far from its "
+                    "tolerance and amplifies round-off by orders of magnitude (the reference measures "
+                    "1e-15 -> 1e-2 for its own C++ port and compares energy only at convergence, "
+                    "tests/test_native.py:90-103); T is compared at convergence in equal_error"}
 
 
 def equal_error_leg(nx=250, ny=125):
@@ -475,6 +535,8 @@ def main():
     ap.add_argument("--adjoint", type=int, default=32, help="also time the square-cylinder adjoint with this many "
                     "cells per body side (32: 47,532 triangles; 100: about 0.47M; 0: skip)")
     ap.add_argument("--sst", type=int, default=0, help="also time k-omega SST on the duct mesh (1) or not (0)")
+    ap.add_argument("--channel", type=int, default=1, help="also time the 3-D channel, 128x64x64 hexes, FP64 "
+                    "against mixed precision with a three-level pressure cycle, 200 outer iterations (1) or not (0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the SIMPLEC path on a single MI355X "
@@ -526,6 +588,8 @@ def main():
         out["adjoint"] = adjoint_leg(args.adjoint, tol=1e-10, max_iter=40000, gmres_tol=1e-8)
     if args.sst:
         out["sst"] = sst_leg(4 * args.nx, args.ny // 2, args.steps, args.warmup)
+    if args.channel:
+        out["channel"] = channel_leg(128, 64, 64, 200)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"], fields = cpu_baseline(args.nx, args.ny, args.cpu_warmup, args.cpu_iters)

@@ -92,6 +92,8 @@ _SIGS = {
     "ftmi_amg_destroy": ([P], None),
     "ftmi_amg_add_level": ([P, I64, I64, I64] + [P] * 14, I32),
     "ftmi_amg_set_coarse": ([P, I64, P, P, P], I32),
+    "ftmi_amg_set_coarse_iter": ([P, I64, I64, P, P, P, P, I32, F64, P, P, P], I32),
+    "ftmi_amg_set_coarse_iter_f32": ([P, I64, I64, P, P, P, P, I32, F64, P, P, P], I32),
     "ftmi_amg_vcycle": ([P, P, P, P], I32),
     "ftmi_amg_create_f32": ([F64, I32], P),
     "ftmi_amg_add_level_f32": ([P, I64, I64, I64] + [P] * 14, I32),

@@ -106,10 +106,16 @@ class SmoothedAggregationAMG:
     debug = False
 
     def __init__(self, A, device="cuda", theta=0.08, max_coarse=300, max_levels=20, sweeps=2, omega=0.8,
-                 singular=False, host_pattern=None, refreshable=False):
+                 singular=False, host_pattern=None, refreshable=False, coarse_sweeps=0):
         self.device = torch.device(device)
         self.refreshable = refreshable  # keep what ``refresh`` needs (the transpose map of each P)
         self.theta, self.max_coarse, self.max_levels = theta, max_coarse, max_levels
+        # coarse_sweeps > 0: when the level count cuts the hierarchy above max_coarse unknowns,
+        # the coarsest system is solved by that many weighted-Jacobi sweeps from zero instead of
+        # a dense inverse (the reference always inverts it densely, amg.py:104-112)
+        self.coarse_sweeps = int(coarse_sweeps)
+        if self.coarse_sweeps == 1:
+            raise ValueError("coarse_sweeps must be 0 (dense inverse) or at least 2")
         self.omega, self.sweeps, self.singular = omega, sweeps, singular
         A0 = A if isinstance(A, DevCSR) else DevCSR.from_scipy(A, self.device)
         if host_pattern is None and not isinstance(A, DevCSR):
@@ -208,10 +214,8 @@ class SmoothedAggregationAMG:
         self.levels, self.aggregates, self.rhos, self._plans, self._masks = [], [], [], [], []
         self._grow(A, host_pattern)
         self._check_overflow()
-        cinv = self._coarse_inverse()
-        self.coarse_n = cinv.shape[0]
-        self.coarse_inv_h = cinv
-        self._upload(cinv)
+        self._prepare_coarse()
+        self._upload()
         self.timing["total"] = time.perf_counter() - t_start
 
     def _reset_timing(self):
@@ -365,15 +369,65 @@ class SmoothedAggregationAMG:
             self._grow(A, (fv.indptr_h, fv.indices_h) if k == 0 else None, first)
             self._check_overflow()
         self.timing["levels_kept"] = k
-        cinv = self._coarse_inverse()
-        self.coarse_inv_h = cinv
-        if k == nlev and len(self.levels) == nlev and cinv.shape[0] == self.coarse_n:
-            self._cinv.copy_(torch.as_tensor(np.ascontiguousarray(cinv), device=self.device))
+        if k == nlev and len(self.levels) == nlev and self.coarse.n == self.coarse_n and \
+                self._iterative() == self._coarse_iterative:
+            self._update_coarse()  # the handle's pointers stay valid
         else:
-            self.coarse_n = cinv.shape[0]
+            self._prepare_coarse()
             self._destroy_handle()
-            self._upload(cinv)
+            self._upload()
         self.timing["total"] = time.perf_counter() - t_start
+
+    # ------------------------------------------------------------------ coarse solve
+    def _iterative(self) -> bool:
+        """Whether the coarsest system is solved by sweeps: requested, below at least one level,
+        and larger than max_coarse (the level count cut the hierarchy)."""
+        return self.coarse_sweeps > 0 and bool(self.levels) and self.coarse.n > self.max_coarse
+
+    def _prepare_coarse(self):
+        """The coarse solve of the current coarsest matrix, in new buffers: its dense inverse
+        (host) or, solved by sweeps, its inverse diagonal (device)."""
+        self._coarse_iterative = self._iterative()
+        c = self.coarse
+        self.coarse_n = c.n
+        if self._coarse_iterative:
+            self.coarse_inv_h = None
+            self._cdiag = torch.empty(c.n, dtype=torch.float64, device=self.device)
+            self._cdinv = torch.empty(c.n, dtype=torch.float64, device=self.device)
+            self._coarse_diag()
+        else:
+            self.coarse_inv_h = self._coarse_inverse()
+            self.coarse_n = self.coarse_inv_h.shape[0]
+
+    def _coarse_diag(self):
+        """D^-1 of the coarsest matrix and the weight of its sweeps, 1 / (1.1 rho(D^-1 A_c)) by
+        the set-up's power iteration: the sweeps then contract every mode (omega lambda < 1), which
+        the smoother's 0.8 does not guarantee on Galerkin operators with rho above 2.5."""
+        L, st, c = nat.lib(), nat.stream_ptr(), self.coarse
+        nat.check(L.ftmi_amgs_diag(c.n, c.nnz, nat.ptr(c.ptr), nat.ptr(c.col), nat.ptr(c.val), nat.ptr(self._cdiag),
+                                   nat.ptr(self._cdinv), st), "diag")
+        x = start_vector_device(c.n, self.device)
+        y = torch.empty(2 * c.n, dtype=torch.float64, device=self.device)
+        rho = torch.empty(2, dtype=torch.float64, device=self.device)
+        nat.check(L.ftmi_amgs_power(c.n, c.nnz, nat.ptr(c.ptr), nat.ptr(c.col), nat.ptr(c.val), nat.ptr(self._cdinv),
+                                    nat.ptr(x), nat.ptr(y), 15, nat.ptr(self._partials), nat.ptr(rho), st), "power")
+        self._comega = 1.0 / (1.1 * max(float(rho[0].item()), 1e-300))
+
+    def _set_coarse_iter(self):
+        c = self.coarse
+        nat.check(nat.lib().ftmi_amg_set_coarse_iter(self.handle, self.coarse_n, c.nnz, nat.ptr(c.ptr), nat.ptr(c.col),
+                                                     nat.ptr(c.val), nat.ptr(self._cdinv), self.coarse_sweeps,
+                                                     float(self._comega), nat.ptr(self._cb), nat.ptr(self._cx),
+                                                     nat.ptr(self._cy)), "amg_set_coarse_iter")
+
+    def _update_coarse(self):
+        """The coarse solve of new coarse values, in place (same size and kind)."""
+        if self._coarse_iterative:
+            self._coarse_diag()
+            self._set_coarse_iter()  # the new weight
+        else:
+            self.coarse_inv_h = self._coarse_inverse()
+            self._cinv.copy_(torch.as_tensor(np.ascontiguousarray(self.coarse_inv_h), device=self.device))
 
     def _coarse_inverse(self) -> np.ndarray:
         Ad = self.coarse.to_scipy().toarray()
@@ -399,9 +453,7 @@ class SmoothedAggregationAMG:
             nat.check(L.ftmi_amgs_diag(A.n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val),
                                        nat.ptr(plan["diag"]), nat.ptr(dinv), st), "diag")
             self._refresh_level(k)
-        cinv = self._coarse_inverse()
-        self.coarse_inv_h = cinv
-        self._cinv.copy_(torch.as_tensor(np.ascontiguousarray(cinv), device=self.device))
+        self._update_coarse()
 
     def _refresh_level(self, k, first=None):
         """Values of level k's P, R, A P and of the next level's matrix from level k's current
@@ -433,7 +485,7 @@ class SmoothedAggregationAMG:
                                                  nat.stream_ptr()), "ldu_to_csr")
         self.refresh()
 
-    def _upload(self, cinv):
+    def _upload(self):
         dev, L = self.device, nat.lib()
         self.handle = L.ftmi_amg_create(float(self.omega), int(self.sweeps))
         self._vecs = []
@@ -445,12 +497,17 @@ class SmoothedAggregationAMG:
                                            nat.ptr(P.ptr), nat.ptr(P.col), nat.ptr(P.val), nat.ptr(R.ptr),
                                            nat.ptr(R.col), nat.ptr(R.val), nat.ptr(dinv), *[nat.ptr(t) for t in v]),
                       "amg_add_level")
-        nc = cinv.shape[0]
-        self._cinv = torch.as_tensor(np.ascontiguousarray(cinv), device=dev)
+        nc = self.coarse_n
         self._cb = torch.zeros(nc, dtype=torch.float64, device=dev)
         self._cx = torch.zeros(nc, dtype=torch.float64, device=dev)
-        nat.check(L.ftmi_amg_set_coarse(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb), nat.ptr(self._cx)),
-                  "amg_set_coarse")
+        if self._coarse_iterative:
+            self._cinv = None
+            self._cy = torch.zeros(nc, dtype=torch.float64, device=dev)
+            self._set_coarse_iter()
+        else:
+            self._cinv = torch.as_tensor(np.ascontiguousarray(self.coarse_inv_h), device=dev)
+            nat.check(L.ftmi_amg_set_coarse(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb),
+                                            nat.ptr(self._cx)), "amg_set_coarse")
 
     def to_fp32(self) -> "AmgFP32":
         """The same hierarchy with FP32 values and vectors (the mixed-precision pressure solve)."""
@@ -509,11 +566,16 @@ class AmgFP32:
                                                nat.ptr(R.col), nat.ptr(rv), nat.ptr(dv), *[nat.ptr(t) for t in v]),
                       "amg_add_level_f32")
         nc = src.coarse_n
-        self._cinv = self._coarse_f32()
         self._cb = torch.zeros(nc, dtype=torch.float32, device=dev)
         self._cx = torch.zeros(nc, dtype=torch.float32, device=dev)
-        nat.check(L.ftmi_amg_set_coarse_f32(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb),
-                                            nat.ptr(self._cx)), "amg_set_coarse_f32")
+        if src._coarse_iterative:
+            self._cval, self._cdinv = to_f32(src.coarse.val), to_f32(src._cdinv)
+            self._cy = torch.zeros(nc, dtype=torch.float32, device=dev)
+            self._set_coarse_iter()
+        else:
+            self._cinv = self._coarse_f32()
+            nat.check(L.ftmi_amg_set_coarse_f32(self.handle, nc, nat.ptr(self._cinv), nat.ptr(self._cb),
+                                                nat.ptr(self._cx)), "amg_set_coarse_f32")
 
     def refresh(self):
         """Round the (refreshed) FP64 values of the source into this twin's FP32 buffers."""
@@ -524,7 +586,20 @@ class AmgFP32:
             to_f32(P.val, pv)
             to_f32(R.val, rv)
             to_f32(dinv, dv)
-        self._cinv.copy_(self._coarse_f32())
+        if self.src._coarse_iterative:
+            to_f32(self.src.coarse.val, self._cval)
+            to_f32(self.src._cdinv, self._cdinv)
+            self._set_coarse_iter()  # the source's new weight
+        else:
+            self._cinv.copy_(self._coarse_f32())
+
+    def _set_coarse_iter(self):
+        src, c = self.src, self.src.coarse
+        nat.check(nat.lib().ftmi_amg_set_coarse_iter_f32(self.handle, src.coarse_n, c.nnz, nat.ptr(c.ptr),
+                                                         nat.ptr(c.col), nat.ptr(self._cval), nat.ptr(self._cdinv),
+                                                         src.coarse_sweeps, float(src._comega), nat.ptr(self._cb),
+                                                         nat.ptr(self._cx), nat.ptr(self._cy)),
+                  "amg_set_coarse_iter_f32")
 
     def _coarse_f32(self) -> torch.Tensor:
         """The coarse inverse in FP32. A regular system takes the FP64 inverse rounded. A singular

@@ -73,6 +73,12 @@ class UnstructuredCase:
     init_velocity: List[float] = field(default_factory=lambda: [0.0, 0.0, 0.0])
     init_T: float = 0.0
     precision: str = "fp64"
+    # pressure AMG (new; the reference fixes them, amg.py:80-112): at most amg_levels levels
+    # counting the coarsest (None: the reference's 20, coarsening to <= 300 unknowns); when that
+    # count leaves the coarsest system above 300 unknowns it is solved by amg_coarse_sweeps
+    # weighted-Jacobi sweeps instead of a dense inverse
+    amg_levels: Optional[int] = None
+    amg_coarse_sweeps: int = 20
     periodic: List[List[str]] = field(default_factory=list)
     periodic_pressure_gradient: List[float] = field(default_factory=lambda: [0.0, 0.0, 0.0])
     # turbulence (case.py:99-110): laminar | k-omega-sst
@@ -99,6 +105,10 @@ class UnstructuredCase:
         if self.precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed' (FP32 transport assembly and Krylov solves, "
                              "FP32 AMG-PCG refined in FP64)")
+        if self.amg_levels is not None and int(self.amg_levels) < 2:
+            raise ValueError("amg_levels counts the coarsest level: at least 2")
+        if int(self.amg_coarse_sweeps) < 2:
+            raise ValueError("amg_coarse_sweeps must be at least 2")
         self.periodic = [list(p) for p in self.periodic]
         t = str(self.turbulence).lower()
         self.turbulence = TURBULENCE_ALIASES.get(t, t)

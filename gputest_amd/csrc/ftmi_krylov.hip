@@ -756,6 +756,15 @@ struct Amg : AmgBase {
     int64_t nc = 0;
     const T* cinv = nullptr;
     T *cb = nullptr, *cx = nullptr;
+    // iterative coarse solve (csweeps > 0, instead of the dense inverse): csweeps weighted-
+    // Jacobi sweeps from zero on the coarsest matrix cA (a hierarchy cut at a level count)
+    int csweeps = 0;
+    double comega = 0.0;  // the coarse sweeps' weight (below 2 / rho(D^-1 A_c), so they converge)
+    CsrOp<T> cA{};
+    const T* cdinv = nullptr;
+    T* cy = nullptr;
+    int tpr_c = 1;
+    int64_t nnz_c = 0;
     double omega = 0.8;
     int sweeps = 2;
     Amg() { bits = 8 * (int)sizeof(T); }
@@ -1026,6 +1035,21 @@ constexpr PcgFinish kNoFinish{nullptr, nullptr, 0, 0.0, nullptr, 0, kNoPost};
 template <typename T>
 static void vcycle(const Amg<T>& M, int k, hipStream_t st, const int* done, double* part, int red = kRedSum,
                    PcgFinish fin = kNoFinish) {
+    if (k == M.nlev && M.csweeps >= 2) {
+        // weighted-Jacobi sweeps from zero: a symmetric positive definite approximate inverse
+        // (a polynomial in D^-1 A times D^-1), so the cycle stays a valid PCG preconditioner
+        TPR_LAUNCH(M.tpr_c, vc_pre2, M.nc, st, M.nc, M.cA, M.cdinv, M.cb, M.cx, M.comega, done, kNoFinish);
+        T* cur = M.cx;
+        T* other = M.cy;
+        for (int s = 2; s < M.csweeps; ++s) {
+            jacobi<T>(M.tpr_c, M.nc, M.cA, M.cdinv, M.cb, cur, other, M.comega, done, nullptr, kRedNone, st);
+            T* t = cur;
+            cur = other;
+            other = t;
+        }
+        if (cur != M.cx) vc_copy<T><<<grid_for(M.nc), kBlock, 0, st>>>(M.nc, cur, M.cx, done);
+        return;
+    }
     if (k == M.nlev) {
         const int rows_per_block = kBlock / kWave;
         vc_coarse<T><<<(int)((M.nc + rows_per_block - 1) / rows_per_block), kBlock, 0, st>>>(M.nc, M.cinv, M.cb,
@@ -1126,6 +1150,30 @@ int amg_set_coarse(void* h, int64_t n, const T* inv, T* vb, T* vx) {
     return 0;
 }
 
+template <typename T>
+int amg_set_coarse_iter(void* h, int64_t n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_col,
+                        const T* a_val, const T* dinv, int sweeps, double omega, T* vb, T* vx, T* vy) {
+    Amg<T>* M = amg_of<T>(h, "ftmi_amg_set_coarse_iter: handle precision does not match the values");
+    if (!M) return -1;
+    if (sweeps < 2 || M->nlev < 1) {
+        set_error_msg("ftmi_amg_set_coarse_iter: needs at least two sweeps and one level above the coarse one");
+        return -1;
+    }
+    M->nc = n;
+    M->lv[M->nlev - 1].tpr_r = pick_tpr(n, M->lv[M->nlev - 1].nnz_p);
+    M->cinv = nullptr;
+    M->csweeps = sweeps;
+    M->comega = omega;
+    M->cA = CsrOp<T>{a_ptr, a_col, a_val};
+    M->nnz_c = nnz;
+    M->tpr_c = pick_tpr(n, nnz);
+    M->cdinv = dinv;
+    M->cb = vb;
+    M->cx = vx;
+    M->cy = vy;
+    return 0;
+}
+
 // the threads-per-row choices of every level again, with the current bias
 template <typename T>
 void amg_retune(Amg<T>* M) {
@@ -1136,6 +1184,7 @@ void amg_retune(Amg<T>* M) {
         const int64_t rows_r = l + 1 < M->nlev ? M->lv[l + 1].n : M->nc;
         L.tpr_r = pick_tpr(rows_r, L.nnz_p);
     }
+    if (M->csweeps > 0) M->tpr_c = pick_tpr(M->nc, M->nnz_c);
 }
 
 template <typename T>
@@ -1230,6 +1279,18 @@ int ftmi_amg_set_coarse(void* h, int64_t n, const double* inv, double* vb, doubl
 
 int ftmi_amg_set_coarse_f32(void* h, int64_t n, const float* inv, float* vb, float* vx) {
     return amg_set_coarse<float>(h, n, inv, vb, vx);
+}
+
+int ftmi_amg_set_coarse_iter(void* h, int64_t n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_col,
+                             const double* a_val, const double* dinv, int sweeps, double omega, double* vb,
+                             double* vx, double* vy) {
+    return amg_set_coarse_iter<double>(h, n, nnz, a_ptr, a_col, a_val, dinv, sweeps, omega, vb, vx, vy);
+}
+
+int ftmi_amg_set_coarse_iter_f32(void* h, int64_t n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_col,
+                                 const float* a_val, const float* dinv, int sweeps, double omega, float* vb,
+                                 float* vx, float* vy) {
+    return amg_set_coarse_iter<float>(h, n, nnz, a_ptr, a_col, a_val, dinv, sweeps, omega, vb, vx, vy);
 }
 
 // x = M^{-1} b with one V-cycle (b, x: level-0 vectors, distinct from the level buffers).

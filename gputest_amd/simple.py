@@ -478,8 +478,10 @@ class SimpleSolver:
                 self.amg_refreshes += 1
             else:
                 if self._amg is None:
+                    lv = {} if c.amg_levels is None else {"max_levels": int(c.amg_levels),
+                                                          "coarse_sweeps": int(c.amg_coarse_sweeps)}
                     self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp,
-                                                               singular=self.singular_p, refreshable=True)
+                                                               singular=self.singular_p, refreshable=True, **lv)
                 else:  # the hierarchy a full set-up gives, keeping the levels whose aggregates stay
                     self._amg.rebuild_ldu(fv, self.diag_p, self.offp)
                 self._amg32 = self._amg.to_fp32() if self.mixed else None

@@ -208,6 +208,13 @@ int ftmi_amg_add_level(void* amg, int64_t n, int64_t nnz_a, int64_t nnz_p, const
                        const int32_t* r_col, const double* r_val, const double* dinv, double* vb, double* vx,
                        double* vy, double* vr);
 int ftmi_amg_set_coarse(void* amg, int64_t n, const double* inv, double* vb, double* vx);
+/* new: instead of a dense inverse, the coarsest system (CSR a_*, nnz entries, inverse diagonal
+ * dinv) is solved by `sweeps` >= 2 Jacobi sweeps from zero with weight omega (below
+ * 2 / rho(D^-1 A) for them to converge), for a hierarchy cut at a level count whose coarsest
+ * matrix is too large for a dense inverse; vb, vx, vy: its vectors */
+int ftmi_amg_set_coarse_iter(void* amg, int64_t n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_col,
+                             const double* a_val, const double* dinv, int sweeps, double omega, double* vb,
+                             double* vx, double* vy);
 int ftmi_amg_vcycle(void* amg, const double* b, double* x, void* stream);
 /* the same hierarchy with FP32 values and vectors (new: the mixed-precision pressure solve);
  * a handle takes only the entry points of its own precision */
@@ -218,6 +225,9 @@ int ftmi_amg_add_level_f32(void* amg, int64_t n, int64_t nnz_a, int64_t nnz_p, c
                            const int32_t* r_col, const float* r_val, const float* dinv, float* vb, float* vx,
                            float* vy, float* vr);
 int ftmi_amg_set_coarse_f32(void* amg, int64_t n, const float* inv, float* vb, float* vx);
+int ftmi_amg_set_coarse_iter_f32(void* amg, int64_t n, int64_t nnz, const int32_t* a_ptr, const int32_t* a_col,
+                                 const float* a_val, const float* dinv, int sweeps, double omega, float* vb, float* vx,
+                                 float* vy);
 int ftmi_amg_vcycle_f32(void* amg, const float* b, float* x, void* stream);
 /* host helper (host pointers): three-pass greedy aggregation of amg.py:26-57 over a strength
  * graph in CSR form; returns the number of aggregates */

@@ -5,6 +5,7 @@ reference's own bar (net imbalance / inflow < 1e-8, ``validation/benchmarks.py:4
 import dataclasses
 
 import numpy as np
+import pytest
 
 import oracle.simplec_oracle as O
 from gputest_amd.batch import CavitySpec, StepSpec, concatenate_meshes, sample_cases, sample_cavities
@@ -83,6 +84,7 @@ def _oracle_case(spec, nu, **kw):
     return O.Case(boundaries=bnd, viscosity=nu, **kw)
 
 
+@pytest.mark.slow
 def test_step_case_conserves_mass_in_the_oracle():
     """The reference's step check: Re = 100, 6 cells per step height, converged to 1e-6; the
     net boundary mass flux over the inflow below 1e-8; a recirculation behind the step."""

@@ -184,6 +184,7 @@ def test_malformed_files_raise(tmp_path):
         read_fluent_mesh(data[: len(data) // 2])  # cut inside the face sections
 
 
+@pytest.mark.slow
 @pytest.mark.parametrize("sides,cd,lr", PINS)
 def test_square_cylinder_re40_oracle_reproduces_the_reference_pins(sqcyl, sides, cd, lr):
     """The oracle on the mesh this reader returns gives the drag coefficient and recirculation

@@ -3,7 +3,8 @@ matrix (``from_ldu``): the reference re-aggregates at every rebuild (``amg.py:80
 rebuild keeps a level's aggregates only where its new strength mask equals the old one, which
 is exactly where a fresh greedy aggregation (a function of the pattern and the mask) gives
 them again. So every level must come out bit-identical to the full set-up, whichever level
-the masks first differ at."""
+the masks first differ at. Also the hierarchy cut at a level count, whose coarsest system is
+solved by sweeps."""
 import numpy as np
 import pytest
 import torch
@@ -95,3 +96,77 @@ def test_rebuild_of_the_solvers_pressure_system_equals_a_full_set_up():
             _assert_identical(s._amg, ref)
             checked += 1
     assert checked >= 2
+
+
+def _host_vcycle(amg, b):
+    """The V(2,2) cycle of amg.py:130-146 in NumPy over the device hierarchy's own matrices,
+    with the coarsest system solved as the hierarchy says: a dense inverse, or `coarse_sweeps`
+    weighted-Jacobi sweeps from zero with the hierarchy's coarse weight."""
+    levels = [(A.to_scipy(), P.to_scipy(), R.to_scipy(), dinv.cpu().numpy()) for A, P, R, dinv in amg.levels]
+    w, nu = amg.omega, amg.sweeps
+    Ac = amg.coarse.to_scipy()
+
+    def coarse(bc):
+        if amg.coarse_inv_h is not None:
+            return amg.coarse_inv_h @ bc
+        dc, wc = 1.0 / Ac.diagonal(), amg._comega
+        x = wc * dc * bc
+        for _ in range(amg.coarse_sweeps - 1):
+            x = x + wc * dc * (bc - Ac @ x)
+        return x
+
+    def cycle(k, bk):
+        if k == len(levels):
+            return coarse(bk)
+        A, P, R, D = levels[k]
+        x = w * D * bk
+        for _ in range(nu - 1):
+            x = x + w * D * (bk - A @ x)
+        x = x + P @ cycle(k + 1, R @ (bk - A @ x))
+        for _ in range(nu):
+            x = x + w * D * (bk - A @ x)
+        return x
+
+    return cycle(0, b)
+
+
+def test_hierarchy_cut_at_three_levels_solves_its_coarsest_system_by_sweeps():
+    """max_levels = 3 on a 200 x 200 Laplacian, whose levels are 40,000 and 6,700 unknowns and
+    whose third (coarsest) one has 835 (the reference would go on to 124): the coarsest system is
+    solved by weighted-Jacobi sweeps instead of a dense inverse. The device V-cycle equals the
+    NumPy cycle over the same matrices, its FP32 twin follows it, and the cycle is a symmetric
+    positive definite preconditioner (PCG converges)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from gputest_amd.fvmesh import DeviceCSR
+    from gputest_amd.krylov import pcg_csr, to_f32
+
+    m = 200
+    T1 = sp.diags([-np.ones(m - 1), 2 * np.ones(m), -np.ones(m - 1)], [-1, 0, 1])
+    A = (sp.kron(T1, sp.identity(m)) + sp.kron(sp.identity(m), T1)).tocsr()
+    A.sort_indices()
+    amg = SmoothedAggregationAMG(A, DEV, max_levels=3, coarse_sweeps=20)
+    assert len(amg.levels) == 2 and amg.coarse.n > amg.max_coarse and amg.coarse_inv_h is None
+    # the weight 1 / (1.1 rho) of the set-up's power estimate (a lower bound of rho): the sweeps
+    # contract every mode (omega lambda_max < 2) and the weight is not below 1 / (1.1 lambda_max)
+    Ac = amg.coarse.to_scipy()
+    dh = sp.diags(1.0 / np.sqrt(Ac.diagonal()))
+    lam = spla.eigsh(dh @ Ac @ dh, k=1, which="LA", return_eigenvectors=False)[0]
+    assert 0.9999 / (1.1 * lam) <= amg._comega < 2.0 / lam, (amg._comega, lam)
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(A.shape[0])
+    z = amg(t(b)).cpu().numpy()
+    zh = _host_vcycle(amg, b)
+    assert np.linalg.norm(z - zh) <= 1e-12 * np.linalg.norm(zh)
+    z32 = amg.to_fp32()(to_f32(t(b))).double().cpu().numpy()
+    assert np.linalg.norm(z32 - zh) <= 1e-5 * np.linalg.norm(zh)
+    y = rng.standard_normal(A.shape[0])
+    zy = amg(t(y)).cpu().numpy()
+    assert abs(y @ z - b @ zy) <= 1e-10 * abs(y @ z)  # symmetric
+    Ad = DeviceCSR(t(A.data), torch.as_tensor(A.indices.astype(np.int32), device=DEV),
+                   torch.as_tensor(A.indptr.astype(np.int32), device=DEV), A.shape)
+    x, info = pcg_csr(Ad, t(b), 1e-10, 200, amg=amg)
+    print("pcg with the three-level cycle", info)
+    assert info.converged
+    assert np.linalg.norm(A @ x.cpu().numpy() - b) <= 1e-9 * np.linalg.norm(b)

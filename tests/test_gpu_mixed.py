@@ -305,6 +305,40 @@ def test_mixed_channel_converges_to_the_references_fp64_solution():
             assert e <= bar, (precision, k, e)
 
 
+def test_mixed_channel_with_a_two_level_cycle_converges_to_the_fp64_solution():
+    """The channel workload with the pressure preconditioner cut at two levels (``amg_levels``:
+    the coarse level, about 600 unknowns here, is solved by Jacobi sweeps, not a dense
+    inverse), in mixed precision, converges to the FP64 solution with the reference's
+    hierarchy within 1e-6: the converged fields do not depend on the preconditioner."""
+    from gputest_amd.mesh import channel_mesh
+    from tests.golden_cases import channel_case
+
+    mesh = channel_mesh(32, 16, 8)
+    out = {}
+    for precision in ("fp64", "mixed"):
+        case = channel_case(C, max_iter=30000)
+        case.relax_p, case.tol = 0.3, (1e-8 if precision == "fp64" else 1e-6)
+        if precision == "mixed":
+            case.precision, case.amg_levels, case.amg_coarse_sweeps = "mixed", 2, 20
+        s = SimpleSolver(mesh, case, DEV)
+        if precision == "mixed":
+            converge_then_settle(s, out["fp64"][2])
+            assert s._amg.coarse_inv_h is None and len(s._amg.levels) == 1
+        else:
+            while not s.finished:
+                s.step()
+            assert s.converged
+        out[precision] = (h(s.U).T, h(s.p), s.iteration)
+    U, p, _ = out["mixed"]
+    U0, p0, it0 = out["fp64"]
+    ub = float(np.abs(U0[:, 0]).mean())
+    errs = {"u": rel_l2(U[:, 0], U0[:, 0]), "vw": float(np.abs(U[:, 1:] - U0[:, 1:]).max()) / ub,
+            "p": float(np.abs((p - p.mean()) - (p0 - p0.mean())).max()) / ub ** 2}
+    print("channel 32x16x8, two-level mixed vs fp64", errs, "fp64 iterations", it0, flush=True)
+    for k, e in errs.items():
+        assert e <= 1e-6, (k, e)
+
+
 def test_mixed_channel_trajectory_before_the_guard():
     """The same channel from rest at the automatic relaxation, as the workload runs it. Its flow
     has a cross-stream mode that grows about threefold per outer iteration from whatever noise

@@ -3,6 +3,7 @@ the reference): the manufactured field and its hand-derived sources against fini
 differences, observed second order of u, v and T (p above first order) on warped triangles, and
 the Green-Gauss pressure force against the least-squares one."""
 import numpy as np
+import pytest
 
 import oracle.simplec_oracle as O
 
@@ -35,6 +36,7 @@ def test_manufactured_solution_is_consistent():
         assert np.abs(a - b).max() < 1e-3 * max(1.0, np.abs(b).max())
 
 
+@pytest.mark.slow
 def test_oracle_is_second_order_on_warped_triangles():
     """``test_verification.py:40-43`` (u, v > 1.8, p > 1.1), with the energy equation on
     (T > 1.8): grids 10, 20, 40."""
@@ -46,6 +48,7 @@ def test_oracle_is_second_order_on_warped_triangles():
     assert p["p"][-1] > 1.1, p
 
 
+@pytest.mark.slow
 def test_least_squares_pressure_force_is_inconsistent_on_triangles():
     """``test_verification.py:46-50``: the Green-Gauss face-pressure force is the default
     because the least-squares one is several times less accurate on triangles."""

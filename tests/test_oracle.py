@@ -204,6 +204,7 @@ def test_oracle_on_the_reference_kernels_follows_the_plain_oracle():
 
 
 @pytest.mark.skipif(not os.path.exists(REF_LIB), reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.slow
 def test_reference_energy_iterate_is_not_reproducible_across_its_own_paths():
     """The heated-duct temperature mid-run is not an iterate-level quantity even for the
     reference: its 60-step energy BiCGSTAB never meets its 1e-3 tolerance there, and the

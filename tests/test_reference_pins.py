@@ -64,6 +64,7 @@ def test_binary_cavity_is_the_uniform_40_by_40_grid():
     np.testing.assert_allclose(np.sort(a.cell_volume), np.sort(b.cell_volume), rtol=1e-12)
 
 
+@pytest.mark.slow
 def test_oracle_runs_the_example_cases():
     """The README's two command lines: the cavity at Re = 100 against Ghia's profiles (the bar
     of ``test_unstructured.py:34``), and the heated duct's mass balance and temperature bounds

@@ -63,6 +63,7 @@ def test_oracle_follows_the_reference_trajectory(build, name, n_iter):
     assert rel(t.yplus, g["yplus"]) < 1e-12
 
 
+@pytest.mark.slow
 def test_oracle_reproduces_the_reference_channel_validation():
     """The reference's SST pin (``tests/test_turbulence.py:53-58`` on
     ``flowtrace/validation/turbulent.py:62-90``): the Re_tau = 392 half channel converges without
