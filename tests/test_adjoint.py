@@ -9,7 +9,12 @@ the pins are the oracle (itself pinned to the reference's golden vectors):
   outer iteration, nothing of ``gputest_amd.adjoint``) along a random direction over all 65
   parameters, at the first and last profile points and in nu, and central finite differences
   (h = 1e-6) of the oracle's converged drag along the random direction and at one profile
-  point."""
+  point (those re-converge the oracle per perturbed parameter vector: marked slow).
+
+The converged base states come from the cache ``tests/golden/adjoint_*_converged.npz``, checked
+against the oracle's own iteration where they are loaded."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -24,6 +29,9 @@ from tests.adjoint_cases import HALF, case, energy_case, small_mesh, u_ctrl0
 NU = 1.0 / 40.0
 
 
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
 def _solve(mesh, u, nu):
     c, prof = case(O, u, nu)
     o = O.Simplec(mesh, c)
@@ -33,11 +41,35 @@ def _solve(mesh, u, nu):
     return o, prof
 
 
+def _converged(mesh, c, name):
+    """The oracle's converged state of a case, from its cache
+    (``tests/golden/make_adjoint_golden.py``), checked to be a fixed point of the oracle's outer
+    iteration: one more iteration moves no field by more than 1e-12 of its size."""
+    g = np.load(os.path.join(GOLD, name))
+
+    def load():
+        o = O.Simplec(mesh, c)
+        st = {k: g[k] for k in ("U", "p", "F", "Fb")}
+        if "T" in g.files:
+            st["T"] = g["T"]
+        o.load_state({**st, "iteration": int(g["iteration"]), "res_continuity_norm": float(g["res_continuity_norm"])})
+        o.set_lagged_gradients(g["gp"], list(g["gU"]), g["gT"] if "gT" in g.files else None)
+        return o
+
+    o, probe = load(), load()
+    probe.case.max_iter = probe.iteration + 1
+    probe.step()
+    for k in ("U", "p", "F", "Fb") + (("T",) if probe.energy else ()):
+        a, b = getattr(probe, k), getattr(o, k)
+        assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), k
+    return o
+
+
 @pytest.fixture(scope="module")
 def base():
     mesh = small_mesh()
-    o, prof = _solve(mesh, u_ctrl0(), NU)
-    return mesh, o, prof
+    c, prof = case(O, u_ctrl0(), NU)
+    return mesh, _converged(mesh, c, "adjoint_drag_converged.npz"), prof
 
 
 def test_outer_iteration_restatement_has_the_oracle_fixed_point(base):
@@ -86,14 +118,13 @@ def test_hand_written_products_equal_autograd(base, heated, energy):
 def test_cpu_adjoint_matches_the_independent_tangent_reference(base):
     """The adjoint gradient against the tangent reference built from the oracle alone, along a
     random direction over all 65 parameters, at profile points 0 and 63 (which no inlet face of
-    this mesh reaches: both derivatives are zero) and in nu; and central differences of the
-    oracle's converged drag along the same random direction."""
+    this mesh reaches: both derivatives are zero) and in nu."""
     mesh, o, prof = base
     rm = CpuResidualMap(solver_view(o), "cylinder", "inlet", prof)
     gu, gnu, J, info = drag_gradient(None, prof, rmap=rm, tol=1e-11)
     grad = np.r_[gu.numpy(), float(gnu)]
-    o2, _ = _solve(mesh, u_ctrl0(), NU)
-    tr = TangentReference(o2, u_ctrl0(), -HALF, HALF)
+    c, _ = case(O, u_ctrl0(), NU)
+    tr = TangentReference(_converged(mesh, c, "adjoint_drag_converged.npz"), u_ctrl0(), -HALF, HALF)
     assert tr.fixed_point_residual() < 1e-12
     d = _direction()
     for name, v in (("random", d), ("k=0", np.eye(65)[0]), ("k=63", np.eye(65)[63]), ("nu", np.eye(65)[64])):
@@ -101,18 +132,28 @@ def test_cpu_adjoint_matches_the_independent_tangent_reference(base):
         a = float(grad @ v)
         print(name, "adjoint", a, "tangent", t, rec)
         assert abs(a - t) <= 1e-5 * max(abs(t), 1e-3 * np.abs(grad).max()), (name, a, t)
-    h = 1e-6
-    u0 = u_ctrl0()
+
+
+@pytest.mark.slow
+def test_cpu_adjoint_matches_central_differences_along_a_random_direction(base):
+    """Central differences (h = 1e-6) of the oracle's converged drag, re-converged from rest for
+    each perturbed parameter vector, along the random direction over all 65 parameters."""
+    mesh, o, prof = base
+    rm = CpuResidualMap(solver_view(o), "cylinder", "inlet", prof)
+    gu, gnu, J, info = drag_gradient(None, prof, rmap=rm, tol=1e-11)
+    grad = np.r_[gu.numpy(), float(gnu)]
+    d, h = _direction(), 1e-6
 
     def drag(beta):
         return _solve(mesh, beta[:64], beta[64])[0].wall_forces()["cylinder"]["total"][0]
 
-    b0 = np.r_[u0, NU]
+    b0 = np.r_[u_ctrl0(), NU]
     fd = (drag(b0 + h * d) - drag(b0 - h * d)) / (2 * h)
     print("random direction: adjoint", float(grad @ d), "central differences", fd)
     assert abs(float(grad @ d) - fd) <= 1e-5 * abs(fd)
 
 
+@pytest.mark.slow
 def test_cpu_adjoint_matches_central_differences(base):
     mesh, o, prof = base
     rm = CpuResidualMap(solver_view(o), "cylinder", "inlet", prof)
@@ -149,8 +190,8 @@ def _solve_energy(mesh, u, T_wall):
 @pytest.fixture(scope="module")
 def heated():
     mesh = small_mesh()
-    o, prof = _solve_energy(mesh, u_ctrl0(), T_WALL)
-    return mesh, o, prof
+    c, prof = energy_case(O, u_ctrl0(), NU, T_WALL)
+    return mesh, _converged(mesh, c, "adjoint_heat_converged.npz"), prof
 
 
 def test_restated_iteration_with_energy_has_the_oracle_fixed_point(heated):
@@ -166,6 +207,7 @@ def test_restated_iteration_with_energy_has_the_oracle_fixed_point(heated):
     assert abs(J - o.heat_flow()["outlet"]) <= 1e-12 * abs(J)
 
 
+@pytest.mark.slow
 def test_cpu_heat_adjoint_matches_central_differences(heated):
     """d(outlet heat flow)/d(inlet control value, cylinder temperature)."""
     mesh, o, prof = heated
