@@ -28,6 +28,7 @@ system meets the tolerance; steps 1-2, 5 and 7 and all state stay FP64.
 from __future__ import annotations
 
 import ctypes
+import gc
 import time
 from collections import defaultdict
 from typing import Dict, List
@@ -337,6 +338,13 @@ class SimpleSolver:
         nat.check(L.ftmi_capture_plan(arr, len(caps), nat.ptr(self.scal[_NSCAL:]), _MAX_SOLVES), "capture plan")
         g = torch.cuda.CUDAGraph()
         prev = L.ftmi_capture_mode(1)
+        # No garbage collection while the stream is captured: objects kept alive only by
+        # reference cycles (a solver and its turbulence model point at each other; graphs,
+        # events and pinned buffers of earlier solvers) would be destroyed inside the capture,
+        # and their destructors' runtime calls are not allowed there (the process aborts);
+        # such garbage waits until the capture has ended.
+        gc_was_on = gc.isenabled()
+        gc.disable()
         try:
             with torch.cuda.graph(g):
                 for k, t in state.items():  # the iteration-start state, for a redo
@@ -344,6 +352,8 @@ class SimpleSolver:
                 self._graph_out = self._launch(rebuild=False)
             used = L.ftmi_capture_slots_used()
         finally:
+            if gc_was_on:
+                gc.enable()
             L.ftmi_capture_mode(prev)
             L.ftmi_capture_plan(None, 0, None, 0)
         expect = len(self._solve_list(*self._graph_out))
