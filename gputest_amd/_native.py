@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftmi.so")
-ABI_VERSION = 105
+ABI_VERSION = 106
 
 P = ctypes.c_void_p
 I32, I64, F64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -123,6 +123,7 @@ _SIGS = {
     "ftmi_amgs_ldu_to_csr": ([I64] + [P] * 8, I32),
     "ftmi_amgs_diag": ([I64, I64] + [P] * 6, I32),
     "ftmi_amgs_strength": ([I64, I64, P, P, P, P, F64, P, P], I32),
+    "ftmi_amgs_strength_cmp": ([I64, I64, P, P, P, P, F64, P, P, P, I32, P], I32),
     "ftmi_amgs_tentative": ([I64] + [P] * 5, I32),
     "ftmi_amgs_power": ([I64, I64, P, P, P, P, P, P, I32, P, P, P], I32),
     "ftmi_amgs_spgemm": ([I32, I64, I64] + [P] * 10 + [F64] + [P] * 8, I32),

@@ -211,6 +211,10 @@ class SmoothedAggregationAMG:
         self._ovf = torch.zeros(4, dtype=torch.int32, device=dev)
         self._bsum = torch.empty(2048, dtype=torch.int64, device=dev)
         self._partials = torch.empty(2048 * 8, dtype=torch.float64, device=dev)
+        # rebuild_ldu's per-level "mask changed" flags (device) and their host copy
+        self._flags = torch.zeros(self.max_levels, dtype=torch.int32, device=dev)
+        self._flags_h = _pinned(self.max_levels, torch.int32).zero_()
+        self._gen = 0
         self.levels, self.aggregates, self.rhos, self._plans, self._masks = [], [], [], [], []
         self._grow(A, host_pattern)
         self._check_overflow()
@@ -223,8 +227,10 @@ class SmoothedAggregationAMG:
         # whole set-up; levels whose aggregates a rebuild kept (equal strength masks)
         self.timing = {"wait_mask": 0.0, "aggregate": 0.0, "total": 0.0, "levels_kept": 0}
 
-    def _strength_mask(self, A: DevCSR):
-        """D, D^-1 and the strength mask (one byte per entry of A) of level matrix A."""
+    def _strength_mask(self, A: DevCSR, compare_level=None):
+        """D, D^-1 and the strength mask (one byte per entry of A) of level matrix A. With
+        ``compare_level`` = k the mask is also compared with level k's kept mask on the device:
+        ``_flags[k]`` then holds ``_gen`` where they differ."""
         L, st = nat.lib(), nat.stream_ptr()
         n = A.n
         diag = torch.empty(n, dtype=torch.float64, device=self.device)
@@ -232,9 +238,25 @@ class SmoothedAggregationAMG:
         nat.check(L.ftmi_amgs_diag(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
                                    nat.ptr(dinv), st), "diag")
         mask = torch.empty(A.nnz, dtype=torch.uint8, device=self.device)
-        nat.check(L.ftmi_amgs_strength(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
-                                       float(self.theta), nat.ptr(mask), st), "strength")
+        if compare_level is None:
+            nat.check(L.ftmi_amgs_strength(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val), nat.ptr(diag),
+                                           float(self.theta), nat.ptr(mask), st), "strength")
+        else:
+            k = compare_level
+            nat.check(L.ftmi_amgs_strength_cmp(n, A.nnz, nat.ptr(A.ptr), nat.ptr(A.col), nat.ptr(A.val),
+                                               nat.ptr(diag), float(self.theta), nat.ptr(mask),
+                                               nat.ptr(self._masks[k]), nat.ptr(self._flags) + 4 * k, self._gen,
+                                               st), "strength (compared)")
         return diag, dinv, mask
+
+    def _changed(self, upto: int):
+        """Which of levels 0..upto-1 have a new strength mask that differs from the kept one
+        (one device-to-host copy of their flags)."""
+        if upto == 0:
+            return []
+        self._flags_h[:upto].copy_(self._flags[:upto], non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        return [int(v) == self._gen for v in self._flags_h[:upto].tolist()]
 
     def _grow(self, A: DevCSR, pattern, first=None):
         """Add levels below the current ones, starting from level matrix A (whose host pattern
@@ -351,15 +373,17 @@ class SmoothedAggregationAMG:
         # on their kept patterns while their new masks are compared with the kept ones on the
         # device (one host read for all of them); a level below the first changed mask has then
         # received values computed on stale aggregates, and is set up anew below.
+        self._gen = self._gen % (2 ** 31 - 1) + 1  # a value no flag holds yet (flags start at 0)
         firsts, flags = [], []
         for k in range(nlev):
-            first = self._strength_mask(self.levels[k][0])
+            first = self._strength_mask(self.levels[k][0], compare_level=k)
             firsts.append(first)
-            flags.append(torch.ne(first[2], self._masks[k]).any())
-            if k == 0 and bool(flags[0]):
+            if k == 0 and self._changed(1)[0]:
+                flags = [True]
                 break
             self._refresh_level(k, first)
-        flags = torch.stack(flags).cpu().tolist() if flags else []
+        else:
+            flags = self._changed(nlev)
         k = flags.index(True) if True in flags else nlev
         first = firsts[k] if k < nlev else None
         if k < nlev or self.coarse.n > self.max_coarse:

@@ -101,20 +101,29 @@ __global__ void __launch_bounds__(kBlock) k_diag(int64_t n, const int32_t* __res
     }
 }
 
+// With old != nullptr the new mask is also compared with old entry by entry, and a wave that
+// finds a difference stores gen into *flag (every such store writes the same value; the
+// caller passes a gen that *flag does not hold yet, so no clearing pass is needed).
 template <int TPR>
 __global__ void __launch_bounds__(kBlock) k_strength(int64_t n, const int32_t* __restrict__ ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val,
                                                      const double* __restrict__ diag, double theta,
-                                                     uint8_t* __restrict__ mask) {
+                                                     uint8_t* __restrict__ mask,
+                                                     const uint8_t* __restrict__ old, int32_t* flag,
+                                                     int32_t gen) {
     LaneRows<TPR> R;
+    bool differs = false;
     for (int64_t i = R.first; i < n; i += R.stride) {
         const double di = fabs(diag[i]);
         for (int32_t j = ptr[i] + R.lane; j < ptr[i + 1]; j += TPR) {
             const int32_t c = col[j];
-            mask[j] = (c != i && fabs(val[j]) >= theta * sqrt(di * fabs(diag[c]))) ? 1 : 0;
+            const uint8_t m = (c != i && fabs(val[j]) >= theta * sqrt(di * fabs(diag[c]))) ? 1 : 0;
+            mask[j] = m;
+            if (old != nullptr && old[j] != m) differs = true;
         }
     }
+    if (old != nullptr && __ballot(differs) != 0ull && (threadIdx.x & 63) == 0) *flag = gen;
 }
 
 // the strength graph as its own CSR (the strong entries of each row in column order): row
@@ -737,8 +746,23 @@ int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t
                        const double* diag, double theta, uint8_t* mask, void* stream) {
     hipStream_t s = (hipStream_t)stream;
     const int tpr = rows_tpr(n, nnz);
-    ROWS_TPR_LAUNCH(tpr, k_strength, rows_grid(n, tpr), s, n, ptr, col, val, diag, theta, mask);
+    ROWS_TPR_LAUNCH(tpr, k_strength, rows_grid(n, tpr), s, n, ptr, col, val, diag, theta, mask,
+                    (const uint8_t*)nullptr, (int32_t*)nullptr, 0);
     FTMI_LAUNCH_CHECK("ftmi_amgs_strength");
+    return 0;
+}
+
+int ftmi_amgs_strength_cmp(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
+                           const double* diag, double theta, uint8_t* mask, const uint8_t* old, int32_t* flag,
+                           int gen, void* stream) {
+    if (old == nullptr || flag == nullptr) {
+        set_error_msg("ftmi_amgs_strength_cmp: old mask and flag are required");
+        return -1;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const int tpr = rows_tpr(n, nnz);
+    ROWS_TPR_LAUNCH(tpr, k_strength, rows_grid(n, tpr), s, n, ptr, col, val, diag, theta, mask, old, flag, gen);
+    FTMI_LAUNCH_CHECK("ftmi_amgs_strength_cmp");
     return 0;
 }
 

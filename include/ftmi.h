@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 105
+#define FTMI_ABI_VERSION 106
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -287,6 +287,13 @@ int ftmi_amgs_diag(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* co
 /* strength mask |a_ij| >= theta sqrt(|a_ii a_jj|), i != j (amg.py:60-66) */
 int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
                        const double* diag, double theta, uint8_t* mask, void* stream);
+/* new: the strength mask as above, compared entry by entry with a previous mask old of the
+ * same pattern; where any entry differs, gen is stored into *flag (device memory). *flag is
+ * never cleared: pass a gen it does not hold (a counter), then gen == *flag means "differs".
+ * Lets an AMG rebuild find the levels whose aggregation would change without a host pass. */
+int ftmi_amgs_strength_cmp(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
+                           const double* diag, double theta, uint8_t* mask, const uint8_t* old, int32_t* flag,
+                           int gen, void* stream);
 /* new: the strength graph of a level as its own CSR: per-row counts of the mask's strong
  * entries, then (after ftmi_scan_i32 of the counts into s_ptr) their columns in order */
 int ftmi_amgs_strong_count(int64_t n, const int32_t* ptr, const uint8_t* mask, int32_t* cnt, void* stream);
