@@ -528,6 +528,7 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
         s.timers[k] = 0.0
     s.amg_setup_seconds = 0.0
     s.amg_rebuilds = 0
+    s.amg_reuses = 0
     s.amg_refreshes = 0
     s.amg_timing = {}
     s.amg_levels_kept = []
@@ -544,11 +545,13 @@ def run_timed(SimpleSolver, mesh, case, warmup, steps, tag):
     dt = time.perf_counter() - t0
     guards = len(s.guard_events) - guard0
     print(f"[bench:{tag}] timed {steps} steps in {dt:.3f} s; stages {dict(s.timers)}; amg set-up "
-          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds + {s.amg_refreshes} refreshes; guard events {guards}; "
+          f"{s.amg_setup_seconds:.3f} s in {s.amg_rebuilds} rebuilds ({s.amg_reuses} reused) + {s.amg_refreshes} refreshes; guard events {guards}; "
           f"pressure iterations {pits}; last {s.last_linear}", file=sys.stderr, flush=True)
     info = {"stage_seconds": {k: round(v, 4) for k, v in s.timers.items()},
             "amg_setup_seconds": round(s.amg_setup_seconds, 3), "amg_rebuilds": s.amg_rebuilds,
             "amg_refreshes": s.amg_refreshes,
+            # rebuilds whose matrix was bit-identical to a kept hierarchy's (used as it is)
+            "amg_reuses": s.amg_reuses,
             "amg_setup_host_seconds": {k: round(v, 4) for k, v in s.amg_timing.items() if k != "levels_kept"},
             "amg_levels_kept": list(s.amg_levels_kept),
             "guard_events": guards, "pressure_iters_total": int(sum(p or 0 for p in pits)),

@@ -124,6 +124,7 @@ _SIGS = {
     "ftmi_amgs_diag": ([I64, I64] + [P] * 6, I32),
     "ftmi_amgs_strength": ([I64, I64, P, P, P, P, F64, P, P], I32),
     "ftmi_amgs_strength_cmp": ([I64, I64, P, P, P, P, F64, P, P, P, I32, P], I32),
+    "ftmi_differs_u64": ([I64, P, P, P, I32, P], I32),
     "ftmi_amgs_tentative": ([I64] + [P] * 5, I32),
     "ftmi_amgs_power": ([I64, I64, P, P, P, P, P, P, I32, P, P, P], I32),
     "ftmi_amgs_spgemm": ([I32, I64, I64] + [P] * 10 + [F64] + [P] * 8, I32),

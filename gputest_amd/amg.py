@@ -212,9 +212,11 @@ class SmoothedAggregationAMG:
         self._bsum = torch.empty(2048, dtype=torch.int64, device=dev)
         self._partials = torch.empty(2048 * 8, dtype=torch.float64, device=dev)
         # rebuild_ldu's per-level "mask changed" flags (device) and their host copy
-        self._flags = torch.zeros(self.max_levels, dtype=torch.int32, device=dev)
-        self._flags_h = _pinned(self.max_levels, torch.int32).zero_()
+        # (slot max_levels: matches_ldu's "values differ" flag)
+        self._flags = torch.zeros(self.max_levels + 1, dtype=torch.int32, device=dev)
+        self._flags_h = _pinned(self.max_levels + 1, torch.int32).zero_()
         self._gen = 0
+        self._stage = None
         self.levels, self.aggregates, self.rhos, self._plans, self._masks = [], [], [], [], []
         self._grow(A, host_pattern)
         self._check_overflow()
@@ -248,6 +250,34 @@ class SmoothedAggregationAMG:
                                                nat.ptr(self._masks[k]), nat.ptr(self._flags) + 4 * k, self._gen,
                                                st), "strength (compared)")
         return diag, dinv, mask
+
+    def _next_gen(self) -> int:
+        self._gen = self._gen % (2 ** 31 - 1) + 1  # a value no flag holds yet (flags start at 0)
+        return self._gen
+
+    def _stage_ldu(self, fv, diag, off) -> DevCSR:
+        """The CSR values of the face-form matrix (diag, off) into the staging buffer, beside the
+        finest matrix's own values (which keep describing the hierarchy). Returns that matrix."""
+        A0 = self.levels[0][0] if self.levels else self.coarse
+        if self._stage is None or self._stage.numel() != A0.nnz:
+            self._stage = torch.empty(A0.nnz, dtype=torch.float64, device=self.device)
+        nat.check(nat.lib().ftmi_amgs_ldu_to_csr(fv.nc, nat.ptr(fv.cptr), nat.ptr(fv.he_slot), nat.ptr(fv.slot_diag),
+                                                 nat.ptr(fv.indptr), nat.ptr(diag), nat.ptr(off),
+                                                 nat.ptr(self._stage), nat.stream_ptr()), "ldu_to_csr")
+        return A0
+
+    def matches_ldu(self, fv, diag, off) -> bool:
+        """Whether the face-form matrix (diag, off) is, bit for bit, the finest matrix this
+        hierarchy was last set up or rebuilt for: then a set-up of it gives this hierarchy
+        again, entry for entry, and it can be used as it is. One device pass, one host read."""
+        L, st = nat.lib(), nat.stream_ptr()
+        A0 = self._stage_ldu(fv, diag, off)
+        gen, slot = self._next_gen(), self.max_levels
+        nat.check(L.ftmi_differs_u64(A0.nnz, nat.ptr(self._stage), nat.ptr(A0.val), nat.ptr(self._flags) + 4 * slot,
+                                     gen, st), "differs")
+        self._flags_h[slot:].copy_(self._flags[slot:], non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        return int(self._flags_h[slot]) != gen
 
     def _changed(self, upto: int):
         """Which of levels 0..upto-1 have a new strength mask that differs from the kept one
@@ -348,41 +378,55 @@ class SmoothedAggregationAMG:
             raise RuntimeError(f"AMG set-up: {ovf[0]} sparse-product rows exceeded the LDS hash capacity "
                                f"(max distinct {ovf[1]}, max candidates {ovf[2]}, e.g. row {ovf[3]})")
 
-    def rebuild_ldu(self, fv, diag, off):
+    def rebuild_ldu(self, fv, diag, off, staged=False, keep=False) -> bool:
         """The hierarchy of a new pressure matrix (face form of ``fv``), equal to a full set-up
         of it (``from_ldu``) value for value: the greedy aggregation is a function of the level's
         pattern and strength mask alone, so wherever a level's new strength mask equals the one
         its aggregates came from, the aggregates (and, below them, every sparsity pattern) are
         those a fresh aggregation would give, and the level's values are recomputed on the kept
         patterns as ``refresh`` does. From the first level whose mask differs, the levels are
-        set up anew. Needs a hierarchy set up with ``refreshable=True``."""
+        set up anew. Needs a hierarchy set up with ``refreshable=True``.
+
+        ``staged``: the new values are already in the staging buffer (``matches_ldu`` left them
+        there). ``keep``: when the finest level's mask changes (every level would be set up
+        anew), leave this hierarchy untouched and return False, so that the caller can set up a
+        new one and keep this one; True otherwise."""
         import time
 
         if not self.refreshable:
             raise RuntimeError("this hierarchy was set up without refreshable=True")
         t_start = time.perf_counter()
         self._reset_timing()
-        L, st = nat.lib(), nat.stream_ptr()
         A0 = self.levels[0][0] if self.levels else self.coarse
-        nat.check(L.ftmi_amgs_ldu_to_csr(fv.nc, nat.ptr(fv.cptr), nat.ptr(fv.he_slot), nat.ptr(fv.slot_diag),
-                                         nat.ptr(fv.indptr), nat.ptr(diag), nat.ptr(off), nat.ptr(A0.val), st),
-                  "ldu_to_csr")
+        if not staged:
+            self._stage_ldu(fv, diag, off)
         nlev = len(self.levels)
         # The finest level's mask is checked first (one host read), since refreshing the whole
         # hierarchy for nothing costs most when it changes. The coarser levels are then refreshed
         # on their kept patterns while their new masks are compared with the kept ones on the
         # device (one host read for all of them); a level below the first changed mask has then
         # received values computed on stale aggregates, and is set up anew below.
-        self._gen = self._gen % (2 ** 31 - 1) + 1  # a value no flag holds yet (flags start at 0)
+        self._next_gen()
         firsts, flags = [], []
         for k in range(nlev):
-            first = self._strength_mask(self.levels[k][0], compare_level=k)
-            firsts.append(first)
-            if k == 0 and self._changed(1)[0]:
-                flags = [True]
-                break
+            if k == 0:
+                first = self._strength_mask(DevCSR(A0.n, A0.ncols, A0.ptr, A0.col, self._stage), compare_level=0)
+                changed = self._changed(1)[0]
+                if changed and keep:
+                    self.timing["total"] = time.perf_counter() - t_start
+                    return False
+                A0.val.copy_(self._stage)
+                firsts.append(first)
+                if changed:
+                    flags = [True]
+                    break
+            else:
+                first = self._strength_mask(self.levels[k][0], compare_level=k)
+                firsts.append(first)
             self._refresh_level(k, first)
         else:
+            if nlev == 0:
+                A0.val.copy_(self._stage)
             flags = self._changed(nlev)
         k = flags.index(True) if True in flags else nlev
         first = firsts[k] if k < nlev else None
@@ -401,6 +445,7 @@ class SmoothedAggregationAMG:
             self._destroy_handle()
             self._upload()
         self.timing["total"] = time.perf_counter() - t_start
+        return True
 
     # ------------------------------------------------------------------ coarse solve
     def _iterative(self) -> bool:

@@ -126,6 +126,17 @@ __global__ void __launch_bounds__(kBlock) k_strength(int64_t n, const int32_t* _
     if (old != nullptr && __ballot(differs) != 0ull && (threadIdx.x & 63) == 0) *flag = gen;
 }
 
+// *flag = gen when the n 64-bit words of a and b differ anywhere (bitwise, so -0.0 and NaN
+// payloads count as differences): "is this exactly the matrix a hierarchy was built for?"
+__global__ void __launch_bounds__(kBlock) k_differs_u64(int64_t n, const uint64_t* __restrict__ a,
+                                                        const uint64_t* __restrict__ b, int32_t* flag,
+                                                        int32_t gen) {
+    bool differs = false;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        if (a[i] != b[i]) differs = true;
+    if (__ballot(differs) != 0ull && (threadIdx.x & 63) == 0) *flag = gen;
+}
+
 // the strength graph as its own CSR (the strong entries of each row in column order): row
 // counts, then (after a scan of the counts) the columns
 __global__ void __launch_bounds__(kBlock) k_strong_count(int64_t n, const int32_t* __restrict__ ptr,
@@ -763,6 +774,18 @@ int ftmi_amgs_strength_cmp(int64_t n, int64_t nnz, const int32_t* ptr, const int
     const int tpr = rows_tpr(n, nnz);
     ROWS_TPR_LAUNCH(tpr, k_strength, rows_grid(n, tpr), s, n, ptr, col, val, diag, theta, mask, old, flag, gen);
     FTMI_LAUNCH_CHECK("ftmi_amgs_strength_cmp");
+    return 0;
+}
+
+int ftmi_differs_u64(int64_t n, const void* a, const void* b, int32_t* flag, int gen, void* stream) {
+    if (n <= 0) return 0;
+    if (a == nullptr || b == nullptr || flag == nullptr) {
+        set_error_msg("ftmi_differs_u64: null buffer");
+        return -1;
+    }
+    k_differs_u64<<<grid_for(n), kBlock, 0, (hipStream_t)stream>>>(n, (const uint64_t*)a, (const uint64_t*)b, flag,
+                                                                     gen);
+    FTMI_LAUNCH_CHECK("ftmi_differs_u64");
     return 0;
 }
 

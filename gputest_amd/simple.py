@@ -116,6 +116,8 @@ class SimpleSolver:
         self._amg_refreshed = 0
         self.amg_timing: Dict[str, float] = {}
         self.amg_levels_kept: List[int] = []  # per rebuild: levels whose aggregates stayed
+        self._amg_kept = None  # (hierarchy, FP32 twin) that the last full set-up replaced
+        self.amg_reuses = 0  # rebuilds that found their matrix in a kept hierarchy
         self.last_linear = {}
         self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
         self._build_boundaries()
@@ -487,19 +489,15 @@ class SimpleSolver:
                 self._amg_refreshed += 1
                 self.amg_refreshes += 1
             else:
-                if self._amg is None:
-                    lv = {} if c.amg_levels is None else {"max_levels": int(c.amg_levels),
-                                                          "coarse_sweeps": int(c.amg_coarse_sweeps)}
-                    self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp,
-                                                               singular=self.singular_p, refreshable=True, **lv)
-                else:  # the hierarchy a full set-up gives, keeping the levels whose aggregates stay
-                    self._amg.rebuild_ldu(fv, self.diag_p, self.offp)
-                self._amg32 = self._amg.to_fp32() if self.mixed else None
+                how = self._next_hierarchy(fv)
                 self._amg_refreshed = 0
-                for k, v in self._amg.timing.items():
-                    self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
-                self.amg_levels_kept.append(int(self._amg.timing["levels_kept"]))
+                if how in ("rebuilt", "new"):
+                    for k, v in self._amg.timing.items():
+                        self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
+                self.amg_levels_kept.append(int(self._amg.timing["levels_kept"]) if how == "rebuilt" else
+                                            len(self._amg.levels) if how in ("current", "kept") else 0)
                 self.amg_rebuilds += 1
+                self.amg_reuses += how in ("current", "kept")
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
         if self.mixed:
@@ -781,6 +779,41 @@ class SimpleSolver:
         self.algo, self.relax_u, self.relax_p = rb.settings.algorithm, rb.settings.relax_u, rb.settings.relax_p
         self._amg_age = 10 ** 9
         return True
+
+    def _new_hierarchy(self, fv):
+        c = self.case
+        lv = {} if c.amg_levels is None else {"max_levels": int(c.amg_levels),
+                                              "coarse_sweeps": int(c.amg_coarse_sweeps)}
+        self._amg = SmoothedAggregationAMG.from_ldu(fv, self.diag_p, self.offp, singular=self.singular_p,
+                                                   refreshable=True, **lv)
+        self._amg32 = self._amg.to_fp32() if self.mixed else None
+
+    def _next_hierarchy(self, fv) -> str:
+        """The AMG hierarchy of the pressure matrix just assembled, as a full set-up of it gives
+        it (``amg.py:80-146`` re-aggregates at every rebuild). Two hierarchies are kept: the
+        current one and the one that the last full set-up replaced. A hierarchy whose finest
+        matrix is, bit for bit, the new one is what a set-up would give again, and is used as it
+        is: "current", or "kept" (the two are swapped); this happens when the stability guard
+        restores a state whose first matrix was seen before. Otherwise the current hierarchy is
+        rebuilt on the levels whose strength masks stay ("rebuilt"), unless the finest mask
+        changes: then a new hierarchy is set up and the current one is kept ("new")."""
+        if self._amg is None:
+            self._new_hierarchy(fv)
+            return "new"
+        cur = (self._amg, self._amg32)
+        if self._amg.matches_ldu(fv, self.diag_p, self.offp):
+            return "current"
+        kept = self._amg_kept
+        if kept is not None and kept[0].matches_ldu(fv, self.diag_p, self.offp):
+            self._amg_kept = cur
+            self._amg, self._amg32 = kept
+            return "kept"
+        if self._amg.rebuild_ldu(fv, self.diag_p, self.offp, staged=True, keep=True):
+            self._amg32 = self._amg.to_fp32() if self.mixed else None
+            return "rebuilt"
+        self._amg_kept = cur
+        self._new_hierarchy(fv)
+        return "new"
 
     # ------------------------------------------------------------------ state, fields, reports
     def diagnostics(self) -> dict:

@@ -294,6 +294,10 @@ int ftmi_amgs_strength(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t
 int ftmi_amgs_strength_cmp(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, const double* val,
                            const double* diag, double theta, uint8_t* mask, const uint8_t* old, int32_t* flag,
                            int gen, void* stream);
+/* new: *flag = gen where the n 64-bit words at a and b differ anywhere (bitwise); *flag is never
+ * cleared (pass a gen it does not hold). Tells whether a pressure matrix is exactly the one an
+ * AMG hierarchy was built for, so that the hierarchy can be reused as it is. */
+int ftmi_differs_u64(int64_t n, const void* a, const void* b, int32_t* flag, int gen, void* stream);
 /* new: the strength graph of a level as its own CSR: per-row counts of the mask's strong
  * entries, then (after ftmi_scan_i32 of the counts into s_ptr) their columns in order */
 int ftmi_amgs_strong_count(int64_t n, const int32_t* ptr, const uint8_t* mask, int32_t* cnt, void* stream);

@@ -98,6 +98,41 @@ def test_rebuild_of_the_solvers_pressure_system_equals_a_full_set_up():
     assert checked >= 2
 
 
+def test_solver_reuses_a_hierarchy_only_for_its_exact_matrix():
+    """The solver's hierarchy bookkeeping (``SimpleSolver._next_hierarchy``) over a fixed
+    sequence of pressure matrices: a hierarchy is used as it is only for the matrix it was set up
+    for, bit for bit; the current one is rebuilt when the finest strength mask stays, and a new
+    one is set up (keeping the current one) when it changes. After every step the hierarchy in
+    use equals a full set-up of the matrix."""
+    from gputest_amd import case as C
+    from gputest_amd.simple import SimpleSolver
+
+    mesh = cartesian_mesh_2d(64, 48, triangulate=True, warp=0.05)
+    s = SimpleSolver(mesh, C.UnstructuredCase(boundaries={"top": C.BoundarySpec("wall", velocity=[1.0, 0.0])},
+                                              viscosity=0.01, max_iter=5, tol=0.0), DEV)
+    fv = s.fv
+    rng = np.random.default_rng(5)
+    a1, a2 = rng.uniform(0.001, 2.0, fv.n_if), rng.uniform(0.001, 2.0, fv.n_if)  # weak entries differ
+    a1n = a1 * (1.0 + 1e-9 * rng.uniform(-1.0, 1.0, fv.n_if))  # every strength test as for a1
+    a2n = a2 * (1.0 + 1e-9 * rng.uniform(-1.0, 1.0, fv.n_if))
+    a1u = a1.copy()
+    a1u[7] = np.nextafter(a1u[7], 3.0)  # one ulp on one face
+    # (current, kept) after each: H1 -; H2 H1; H1 H2; H1 H2; H1u H2; H2 H1u; H1n H2; H2 H1n; H2n H1n; H1n H2n
+    seq = [(a1, "new"), (a2, "new"), (a1, "kept"), (a1, "current"), (a1u, "rebuilt"), (a2, "kept"),
+           (a1n, "new"), (a2, "kept"), (a2n, "rebuilt"), (a1n, "kept")]
+    got = []
+    for a, want in seq:
+        diag, off = _laplacian(fv, a)
+        s.diag_p.copy_(diag)
+        s.offp.copy_(off)
+        how = s._next_hierarchy(fv)
+        got.append(how)
+        _assert_identical(s._amg, SmoothedAggregationAMG.from_ldu(fv, diag, off, singular=s.singular_p))
+        assert s._amg.matches_ldu(fv, s.diag_p, s.offp)
+    print("hierarchy outcomes", got)
+    assert got == [w for _, w in seq]
+
+
 def _host_vcycle(amg, b):
     """The V(2,2) cycle of amg.py:130-146 in NumPy over the device hierarchy's own matrices,
     with the coarsest system solved as the hierarchy says: a dense inverse, or `coarse_sweeps`
