@@ -31,7 +31,7 @@ import ctypes
 import gc
 import time
 from collections import defaultdict
-from typing import Dict, List
+from typing import Dict, List, Optional
 
 import numpy as np
 import torch
@@ -59,6 +59,21 @@ _NSCAL = 22
 # after the reduced scalars so that one host read returns both: momentum components,
 # pressure, energy, SST k and omega
 _MAX_SOLVES = 8
+
+
+class _PendingSolve:
+    """A Krylov solve still running on the energy stream: its report (ok, iterations,
+    converged, residual, as the capture mode writes it) is copied to pinned host memory,
+    readable once ``done`` has passed. ``targets`` are the iteration's last_linear and
+    diagnostics capture, which receive the info."""
+
+    def __init__(self, report_h: torch.Tensor, done):
+        self.report_h, self.done, self.targets = report_h, done, None
+
+    def result(self) -> KrylovInfo:
+        self.done.synchronize()
+        _, it, conv, res = (float(v) for v in self.report_h.tolist())
+        return KrylovInfo(int(it), res, conv == 1.0)
 
 
 def auto_relax_p(non_orthogonality_deg: float) -> float:
@@ -119,6 +134,12 @@ class SimpleSolver:
         self._amg_kept = None  # (hierarchy, FP32 twin) that the last full set-up replaced
         self.amg_reuses = 0  # rebuilds that found their matrix in a kept hierarchy
         self.last_linear = {}
+        # The energy step of an iteration runs on a stream of its own, beside the next
+        # iteration's flow work, when T feeds nothing back into the flow (no turbulence model,
+        # eager steps, one case): ``overlap_energy`` switches it off.
+        self.overlap_energy = True
+        self._estream = None
+        self._energy_pending = None
         self.relax_p = float(case.relax_p) if case.relax_p is not None else auto_relax_p(fv.non_orthogonality_deg)
         self._build_boundaries()
         self._alloc()
@@ -198,7 +219,7 @@ class SimpleSolver:
         z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)  # noqa: E731
         self.U, self.p = z(dim, nc), z(nc)
         self.F, self.Fb = z(nf), z(nb)
-        self.T = z(nc) if self.energy else None
+        self._T = z(nc) if self.energy else None
         self.ub, self.pb = z(dim, nb), z(nb)
         self.gp_ls, self.gp, self.gU = z(nc, dim), z(nc, dim), z(dim, nc, dim)
         self.diag, self.diag_r, self.inv_r, self.off = z(nc), z(nc), z(nc), z(2 * nf)
@@ -480,7 +501,11 @@ class SimpleSolver:
         # 6. pressure correction
         if rebuild and (self._amg is None or self._amg_age >= self.amg_rebuild):
             t0 = time.perf_counter()
-            if self.mixed and self._amg is not None and self._amg_refreshed < self.mixed_amg_refreshes:
+            reused = self._reuse_hierarchy(fv) if self._amg is not None else None
+            if reused is not None:  # the hierarchy of exactly this matrix is at hand
+                self.amg_levels_kept.append(len(self._amg.levels))
+                self.amg_reuses += 1
+            elif self.mixed and self._amg is not None and self._amg_refreshed < self.mixed_amg_refreshes:
                 # mixed precision: the hierarchy is only the FP32 preconditioner, so its values
                 # are refreshed on the kept aggregates and patterns (no aggregation, no host
                 # reads but the coarse matrix); a full set-up every few refreshes
@@ -491,13 +516,10 @@ class SimpleSolver:
             else:
                 how = self._next_hierarchy(fv)
                 self._amg_refreshed = 0
-                if how in ("rebuilt", "new"):
-                    for k, v in self._amg.timing.items():
-                        self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
-                self.amg_levels_kept.append(int(self._amg.timing["levels_kept"]) if how == "rebuilt" else
-                                            len(self._amg.levels) if how in ("current", "kept") else 0)
+                for k, v in self._amg.timing.items():
+                    self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
+                self.amg_levels_kept.append(int(self._amg.timing["levels_kept"]) if how == "rebuilt" else 0)
                 self.amg_rebuilds += 1
-                self.amg_reuses += how in ("current", "kept")
             self._amg_age = 0
             self.amg_setup_seconds += time.perf_counter() - t0
         if self.mixed:
@@ -517,7 +539,10 @@ class SimpleSolver:
         # 8. energy
         tinfo = None
         if self.energy:
-            tinfo = self._energy_step(L, s, M, B, R)
+            if self._overlap_energy():
+                tinfo = self._energy_step_overlapped(L, M, B)
+            else:
+                tinfo = self._energy_step(L, s, M, B, R)
         self._mark("correct+energy")
         if self.turb is not None:  # simple.py:662-669
             self.turb.solve(L, M, B, s, self.scal[_S_TURB:], self.scal[_S_TURB + 3:])
@@ -547,17 +572,23 @@ class SimpleSolver:
             res["omega"] = float(sc[_S_TURB + 3]) / max(float(sc[_S_TURB + 4]), 1e-300)
         self.iteration += 1
         self.step_count = self.iteration
-        self.last_linear = {"momentum_iters": [i.iterations for i in infos], "pressure_iters": pinfo.iterations,
-                            "pressure_residual": pinfo.residual}
+        if self._energy_pending is not None:  # the previous iteration's solve (done by now)
+            self._resolve_energy()
+        lin = {"momentum_iters": [i.iterations for i in infos], "pressure_iters": pinfo.iterations,
+               "pressure_residual": pinfo.residual}
         if self.mixed:
-            self.last_linear["pressure_rounds"] = pinfo.rounds
-        if tinfo is not None:
-            self.last_linear["energy_iters"] = tinfo.iterations
+            lin["pressure_rounds"] = pinfo.rounds
         if self.turb is not None and self.turb.last_info:
-            self.last_linear["turbulence_iters"] = [i.iterations for i in self.turb.last_info.values()]
+            lin["turbulence_iters"] = [i.iterations for i in self.turb.last_info.values()]
         # what the diagnostics of this iteration need besides the device systems it left behind
         self._capture = {"mom_info": infos, "p_info": pinfo, "energy_info": tinfo, "iteration": self.iteration,
                          "alpha_u": float(self.relax_u), "alpha_T": float(c.relax_T), "rolled_back": False}
+        if isinstance(tinfo, _PendingSolve):  # still running on the energy stream
+            tinfo.targets = (lin, self._capture)
+            self._energy_pending = tinfo
+        elif tinfo is not None:
+            lin["energy_iters"] = tinfo.iterations
+        self._last_linear = lin
         prev = getattr(self, "_last_res", {}), getattr(self, "_last_raw", {})
         self._last_raw = dict(res)
         if self.iteration <= 5:
@@ -578,44 +609,167 @@ class SimpleSolver:
             self.converged = True
         return 1.0
 
-    def _energy_step(self, L, s, M, B, R):
+    def _energy_assemble(self, L, s, M, B, R, buf):
+        """The energy step up to its solve (``simple.py:778-813``): T's boundary values with the
+        lagged gradient, T's gradient, and the relaxed energy system with its residual sums.
+        ``buf`` holds the system buffers the momentum solves do not share (the overlapped step
+        gives the energy solve buffers of its own)."""
         c = self.case
-        self.T_old.copy_(self.T)
-        nat.check(L.ftmi_energy_bvals(M, B, nat.ptr(self.T), nat.ptr(self.gT) if self._has_lagT else None,
+        T = self._T
+        self.T_old.copy_(T)
+        nat.check(L.ftmi_energy_bvals(M, B, nat.ptr(T), nat.ptr(self.gT) if self._has_lagT else None,
                                       nat.ptr(self.Fb), nat.ptr(self.Tb), s), "energy bvals")
         arr = ctypes.c_void_p * 1
-        nat.check(L.ftmi_ls_gradients(M, 1, arr(self.T.data_ptr()), arr(self.Tb.data_ptr()),
+        nat.check(L.ftmi_ls_gradients(M, 1, arr(T.data_ptr()), arr(self.Tb.data_ptr()),
                                       arr(self.gT.data_ptr()), None, s), "T gradient")
         self._has_lagT = True
         a = nat.FtmiAssembleArgs()
         a.F, a.Fb, a.gamma, a.energy, a.nfields = nat.ptr(self.F), nat.ptr(self.Fb), float(c.conductivity / c.cp), 1, 1
-        a.phi[0], a.grad[0], a.phib[0] = nat.ptr(self.T), nat.ptr(self.gT), nat.ptr(self.Tb)
+        a.phi[0], a.grad[0], a.phib[0] = nat.ptr(T), nat.ptr(self.gT), nat.ptr(self.Tb)
         a.b[0], a.x0[0] = nat.ptr(self.bT), nat.ptr(self.Tnew)
         a.relax, a.second_order, a.simplec, a.cp = float(c.relax_T), int(c.scheme == "second-order"), -1, float(c.cp)
         a.diag, a.diag_r, a.off = nat.ptr(self.diagT), nat.ptr(self.diagT_r), nat.ptr(self.offT)
-        a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(self.inv_r)
+        a.resid, a.inv_r = nat.ptr(self.scal[_S_EN:]), nat.ptr(buf["inv_r"])
         if self._parts is not None:
             a.cell_resid = nat.ptr(self._parts["en"])
         if self.mixed:
-            self._fp32_outputs(a, 1)
+            a.out_fp32 = 1
+            a.diag32, a.diag_r32, a.off32 = nat.ptr(buf["diag32"]), nat.ptr(buf["diag_r32"]), nat.ptr(buf["off32"])
+            a.inv_r32, a.ex32, a.b32[0] = nat.ptr(buf["inv_r32"]), nat.ptr(buf["ex32"]), nat.ptr(buf["b32"])
         if self.turb is not None:  # k/cp + mu_t / Pr_t (simple.py:781-782)
             ge_f, ge_b = self.turb.energy_terms(L, M, B, s, c.conductivity / c.cp)
             a.gam_f, a.gam_b = nat.ptr(ge_f), nat.ptr(ge_b)
         if self.en_src is not None:  # simple.py:795-796
             a.extra[0] = nat.ptr(self.en_src)
         nat.check(L.ftmi_transport_assemble(M, B, ctypes.byref(a), R, s), "energy assembly")
+
+    def _energy_solve(self, buf):
+        """The energy system's Jacobi-BiCGSTAB (tol 1e-3, 60 iterations) into T."""
         if self.mixed:
-            if self.soff32 is not None:
-                self.fv.slice_values(self.off32, self.soff32)
-            info = bicgstab_ldu_mixed(self.fv, self.ex32, self.off32, self.b32[0], self.inv_r32, self.Tnew, 1e-3, 60,
-                                      True, ws=self.ws, b_is_residual=True, sliced_off=self.soff32)
+            if buf["soff32"] is not None:
+                self.fv.slice_values(buf["off32"], buf["soff32"])
+            info = bicgstab_ldu_mixed(self.fv, buf["ex32"], buf["off32"], buf["b32"], buf["inv_r32"], self.Tnew,
+                                      1e-3, 60, True, ws=buf["ws"], b_is_residual=True, sliced_off=buf["soff32"])
         else:
-            if self.soff is not None:
-                self.fv.slice_values(self.offT, self.soff)
-            info = bicgstab_ldu(self.fv, self.diagT_r, self.offT, self.bT, self.inv_r, self.Tnew, 1e-3, 60, True,
-                                ws=self.ws, sliced_off=self.soff)
-        self.T.copy_(self.Tnew)  # a copy, not a swap: recorded graphs keep their pointers
+            if buf["soff"] is not None:
+                self.fv.slice_values(self.offT, buf["soff"])
+            info = bicgstab_ldu(self.fv, self.diagT_r, self.offT, self.bT, buf["inv_r"], self.Tnew, 1e-3, 60, True,
+                                ws=buf["ws"], sliced_off=buf["soff"])
+        self._T.copy_(self.Tnew)  # a copy, not a swap: recorded graphs keep their pointers
         return info
+
+    def _energy_step(self, L, s, M, B, R):
+        self._join_energy()  # an earlier overlapped step may still be on the energy stream
+        buf = {"inv_r": self.inv_r, "ws": self.ws, "soff": self.soff}
+        if self.mixed:
+            buf.update(diag32=self.diag32, diag_r32=self.diag_r32, off32=self.off32, inv_r32=self.inv_r32,
+                       ex32=self.ex32, b32=self.b32[0], soff32=self.soff32)
+        self._energy_assemble(L, s, M, B, R, buf)
+        return self._energy_solve(buf)
+
+    # ------------------------------------------------------------------ energy beside the flow
+    def _overlap_energy(self) -> bool:
+        """Whether this iteration's energy step may run beside the next iteration's flow work:
+        T must feed nothing back into the flow (no turbulence model, whose viscosity the
+        energy diffusivity reads and the turbulence update rewrites), the step must be eager
+        (graph recording captures one stream) and the solver must hold one case."""
+        return (self.energy and self.overlap_energy and self.turb is None and not self.use_graph
+                and self._parts is None and self._pin is None)
+
+    def _energy_side(self):
+        """The energy stream (default priority) and the energy solve's own buffers: inverse
+        diagonal, slice-layout operator, reduction partials, Krylov workspace and report."""
+        if self._estream is None:
+            dev, nc, nf = self.device, self.nc, self.fv.n_if
+            z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)  # noqa: E731
+            f = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
+            sl = self.fv.sl_size if self.fv.sl_base is not None else None
+            buf = {"inv_r": z(nc), "ws": Workspace(dev), "soff": z(sl) if sl is not None and not self.mixed else None,
+                   "red_partials": z(2048 * 8), "red_counter": torch.zeros(64, dtype=torch.int32, device=dev),
+                   "report": z(4), "report_h": torch.zeros(2, 4, dtype=torch.float64, pin_memory=True),
+                   "slot": 0}
+            if self.mixed:
+                buf.update(diag32=f(nc), diag_r32=f(nc), off32=f(2 * nf), inv_r32=f(nc), ex32=f(nc), b32=f(nc),
+                           soff32=f(sl) if sl is not None else None)
+            r = nat.FtmiRed()
+            r.partials, r.counter = nat.ptr(buf["red_partials"]), nat.ptr(buf["red_counter"])
+            buf["c_red"] = r
+            self._ebuf = buf
+            self._estream = torch.cuda.Stream(device=dev)
+        return self._estream, self._ebuf
+
+    def _energy_step_overlapped(self, L, M, B):
+        """The energy step on the energy stream, after this iteration's corrections. The main
+        stream waits only for the assembly (which reads the corrected fluxes and writes the
+        energy residual that the iteration's host read takes); the solve, every iteration of
+        it enqueued without host reads (the capture mode of the Krylov solvers, one report
+        slot), runs on while the host starts the next iteration. Same kernels, same operands,
+        same results as the serial step; its Krylov info is read when it is first asked for."""
+        main = torch.cuda.current_stream(self.device)
+        es, buf = self._energy_side()
+        es.wait_stream(main)  # the fluxes and fields of this iteration's corrections
+        with torch.cuda.stream(es):
+            s = nat.stream_ptr()
+            self._energy_assemble(L, s, M, B, ctypes.byref(buf["c_red"]), buf)
+            assembled = es.record_event()
+            caps = (ctypes.c_int64 * 1)(0)  # 0: the full 60 iterations
+            nat.check(L.ftmi_capture_plan(caps, 1, nat.ptr(buf["report"]), 1), "energy capture plan")
+            prev = L.ftmi_capture_mode(1)
+            try:
+                self._energy_solve(buf)
+            finally:
+                L.ftmi_capture_mode(prev)
+                L.ftmi_capture_plan(None, 0, None, 0)
+            # two host slots: the previous iteration's report is read (in that iteration's
+            # successor's host part) while this one may still be on its way
+            slot = buf["slot"]
+            buf["slot"] ^= 1
+            buf["report_h"][slot].copy_(buf["report"], non_blocking=True)
+            done = es.record_event()
+        main.wait_event(assembled)
+        return _PendingSolve(buf["report_h"][slot], done)
+
+    def _resolve_energy(self):
+        """Wait for the pending energy solve and put its Krylov info where the serial step
+        would have put it (the iteration's last_linear and diagnostics capture)."""
+        p, self._energy_pending = self._energy_pending, None
+        info = p.result()
+        lin, cap = p.targets
+        lin["energy_iters"] = info.iterations
+        cap["energy_info"] = info
+
+    def _join_energy(self):
+        """Order the current stream after the energy stream's work (a device-side wait)."""
+        if self._estream is not None:
+            torch.cuda.current_stream(self.device).wait_stream(self._estream)
+
+    @property
+    def T(self):
+        """The temperature field. With the energy step beside the flow, the current stream
+        first waits for the energy stream, so every reader sees the finished solve."""
+        self._join_energy()
+        return self._T
+
+    @T.setter
+    def T(self, value):
+        self._T = value
+
+    @property
+    def last_linear(self) -> dict:
+        """Inner iteration counts of the latest iteration (the energy solve's once finished)."""
+        if self._energy_pending is not None:
+            self._resolve_energy()
+        return self._last_linear
+
+    @last_linear.setter
+    def last_linear(self, value):
+        self._last_linear = value
+
+    @property
+    def last_pressure_iters(self) -> int:
+        """The latest iteration's pressure PCG count, without waiting for an energy solve that
+        is still running beside the flow (``last_linear`` waits for it)."""
+        return int(self._last_linear.get("pressure_iters") or 0)
 
     def set_reference_cells(self, cells):
         """Fix the pressure correction to zero at ``cells`` (one per closed case of a batch): their
@@ -788,18 +942,12 @@ class SimpleSolver:
                                                    refreshable=True, **lv)
         self._amg32 = self._amg.to_fp32() if self.mixed else None
 
-    def _next_hierarchy(self, fv) -> str:
-        """The AMG hierarchy of the pressure matrix just assembled, as a full set-up of it gives
-        it (``amg.py:80-146`` re-aggregates at every rebuild). Two hierarchies are kept: the
-        current one and the one that the last full set-up replaced. A hierarchy whose finest
-        matrix is, bit for bit, the new one is what a set-up would give again, and is used as it
-        is: "current", or "kept" (the two are swapped); this happens when the stability guard
-        restores a state whose first matrix was seen before. Otherwise the current hierarchy is
-        rebuilt on the levels whose strength masks stay ("rebuilt"), unless the finest mask
-        changes: then a new hierarchy is set up and the current one is kept ("new")."""
-        if self._amg is None:
-            self._new_hierarchy(fv)
-            return "new"
+    def _reuse_hierarchy(self, fv) -> Optional[str]:
+        """Two AMG hierarchies are kept: the current one and the one that the last full set-up
+        replaced. One whose finest matrix is, bit for bit, the pressure matrix just assembled is
+        what a set-up (or, in mixed precision, a refresh) of it would give again, and is used as
+        it is: "current", or "kept" (the two are swapped). This happens when the stability guard
+        restores a state whose first matrix was seen before. None when neither matches."""
         cur = (self._amg, self._amg32)
         if self._amg.matches_ldu(fv, self.diag_p, self.offp):
             return "current"
@@ -808,10 +956,21 @@ class SimpleSolver:
             self._amg_kept = cur
             self._amg, self._amg32 = kept
             return "kept"
-        if self._amg.rebuild_ldu(fv, self.diag_p, self.offp, staged=True, keep=True):
+        return None
+
+    def _next_hierarchy(self, fv) -> str:
+        """The AMG hierarchy of the pressure matrix just assembled, as a full set-up of it gives
+        it (``amg.py:80-146`` re-aggregates at every rebuild), after ``_reuse_hierarchy`` found
+        none: the current hierarchy is rebuilt on the levels whose strength masks stay
+        ("rebuilt"), unless the finest mask
+        changes: then a new hierarchy is set up and the current one is kept ("new")."""
+        if self._amg is None:
+            self._new_hierarchy(fv)
+            return "new"
+        if self._amg.rebuild_ldu(fv, self.diag_p, self.offp, keep=True):
             self._amg32 = self._amg.to_fp32() if self.mixed else None
             return "rebuilt"
-        self._amg_kept = cur
+        self._amg_kept = (self._amg, self._amg32)
         self._new_hierarchy(fv)
         return "new"
 
@@ -821,6 +980,9 @@ class SimpleSolver:
         systems it left on the device (``simple.py:diagnostics``, ``diagnostics.py:214-360``)."""
         from .diagnostics import evaluate
 
+        if self._energy_pending is not None:
+            self._resolve_energy()
+        self._join_energy()  # the energy system and T_old of an overlapped energy step
         return evaluate(self)
 
     def residuals(self) -> dict:
