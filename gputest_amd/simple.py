@@ -712,14 +712,7 @@ class SimpleSolver:
             s = nat.stream_ptr()
             self._energy_assemble(L, s, M, B, ctypes.byref(buf["c_red"]), buf)
             assembled = es.record_event()
-            caps = (ctypes.c_int64 * 1)(0)  # 0: the full 60 iterations
-            nat.check(L.ftmi_capture_plan(caps, 1, nat.ptr(buf["report"]), 1), "energy capture plan")
-            prev = L.ftmi_capture_mode(1)
-            try:
-                self._energy_solve(buf)
-            finally:
-                L.ftmi_capture_mode(prev)
-                L.ftmi_capture_plan(None, 0, None, 0)
+            self._energy_solve_unread(L, buf)
             # two host slots: the previous iteration's report is read (in that iteration's
             # successor's host part) while this one may still be on its way
             slot = buf["slot"]
@@ -728,6 +721,18 @@ class SimpleSolver:
             done = es.record_event()
         main.wait_event(assembled)
         return _PendingSolve(buf["report_h"][slot], done)
+
+    def _energy_solve_unread(self, L, buf):
+        """The energy solve with every iteration enqueued and no host read (the Krylov
+        solvers' capture mode, one report slot)."""
+        caps = (ctypes.c_int64 * 1)(0)  # 0: the full 60 iterations
+        nat.check(L.ftmi_capture_plan(caps, 1, nat.ptr(buf["report"]), 1), "energy capture plan")
+        prev = L.ftmi_capture_mode(1)
+        try:
+            self._energy_solve(buf)
+        finally:
+            L.ftmi_capture_mode(prev)
+            L.ftmi_capture_plan(None, 0, None, 0)
 
     def _resolve_energy(self):
         """Wait for the pending energy solve and put its Krylov info where the serial step
