@@ -129,6 +129,10 @@ class SimpleSolver:
         self.mixed_amg_refreshes = 5
         self.amg_refreshes = 0
         self._amg_refreshed = 0
+        # a refresh keeps the aggregates of an older matrix: after a guard rollback (new
+        # relaxation, possibly SIMPLE instead of SIMPLEC) or a pressure solve that missed its
+        # tolerance, the next rebuild is a full set-up instead
+        self._amg_needs_setup = False
         self.amg_timing: Dict[str, float] = {}
         self.amg_levels_kept: List[int] = []  # per rebuild: levels whose aggregates stayed
         self._amg_kept = None  # (hierarchy, FP32 twin) that the last full set-up replaced
@@ -505,7 +509,8 @@ class SimpleSolver:
             if reused is not None:  # the hierarchy of exactly this matrix is at hand
                 self.amg_levels_kept.append(len(self._amg.levels))
                 self.amg_reuses += 1
-            elif self.mixed and self._amg is not None and self._amg_refreshed < self.mixed_amg_refreshes:
+            elif (self.mixed and self._amg is not None and self._amg_refreshed < self.mixed_amg_refreshes
+                  and not self._amg_needs_setup):
                 # mixed precision: the hierarchy is only the FP32 preconditioner, so its values
                 # are refreshed on the kept aggregates and patterns (no aggregation, no host
                 # reads but the coarse matrix); a full set-up every few refreshes
@@ -516,6 +521,7 @@ class SimpleSolver:
             else:
                 how = self._next_hierarchy(fv)
                 self._amg_refreshed = 0
+                self._amg_needs_setup = False
                 for k, v in self._amg.timing.items():
                     self.amg_timing[k] = self.amg_timing.get(k, 0.0) + v
                 self.amg_levels_kept.append(int(self._amg.timing["levels_kept"]) if how == "rebuilt" else 0)
@@ -527,6 +533,8 @@ class SimpleSolver:
             self.pp.zero_()
             pinfo = pcg_ldu_mixed(fv, self.diag_p, self.offp, self.offp32, self.rhs_p, self.pp, c.pressure_tol, 200,
                                   self._amg32, self.singular_p, self.max_refine, ws=self.ws)
+            if pinfo.iterations >= 0 and not pinfo.converged:  # (capture mode reports later)
+                self._amg_needs_setup = True
         else:
             pinfo = pcg_ldu(fv, self.diag_p, self.offp, self.rhs_p, self.pp, c.pressure_tol, 200, self._amg,
                             self.singular_p, ws=self.ws)
@@ -937,6 +945,7 @@ class SimpleSolver:
         self.iteration = self.step_count = at
         self.algo, self.relax_u, self.relax_p = rb.settings.algorithm, rb.settings.relax_u, rb.settings.relax_p
         self._amg_age = 10 ** 9
+        self._amg_needs_setup = True
         return True
 
     def _new_hierarchy(self, fv):
