@@ -689,21 +689,32 @@ class SimpleSolver:
         diagonal, slice-layout operator, reduction partials, Krylov workspace and report."""
         if self._estream is None:
             dev, nc, nf = self.device, self.nc, self.fv.n_if
+            es = torch.cuda.Stream(device=dev)
             z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)  # noqa: E731
             f = lambda *s: torch.zeros(*s, dtype=torch.float32, device=dev)  # noqa: E731
             sl = self.fv.sl_size if self.fv.sl_base is not None else None
-            buf = {"inv_r": z(nc), "ws": Workspace(dev), "soff": z(sl) if sl is not None and not self.mixed else None,
-                   "red_partials": z(2048 * 8), "red_counter": torch.zeros(64, dtype=torch.int32, device=dev),
-                   "report": z(4), "report_h": torch.zeros(2, 4, dtype=torch.float64, pin_memory=True),
-                   "slot": 0}
-            if self.mixed:
-                buf.update(diag32=f(nc), diag_r32=f(nc), off32=f(2 * nf), inv_r32=f(nc), ex32=f(nc), b32=f(nc),
-                           soff32=f(sl) if sl is not None else None)
+            # allocated on the energy stream, so that the caching allocator never hands their
+            # memory to the main stream while energy kernels may still use it
+            with torch.cuda.stream(es):
+                buf = {"inv_r": z(nc), "ws": Workspace(dev),
+                       "soff": z(sl) if sl is not None and not self.mixed else None,
+                       "red_partials": z(2048 * 8), "red_counter": torch.zeros(64, dtype=torch.int32, device=dev),
+                       "report": z(4)}
+                if self.mixed:
+                    buf.update(diag32=f(nc), diag_r32=f(nc), off32=f(2 * nf), inv_r32=f(nc), ex32=f(nc), b32=f(nc),
+                               soff32=f(sl) if sl is not None else None)
+            buf.update(report_h=torch.zeros(2, 4, dtype=torch.float64, pin_memory=True), slot=0)
+            # the main stream's tensors that the energy stream reads or writes: freed (with the
+            # solver) while an energy solve is still running, their memory waits for it
+            for t in [self._T, self.Tnew, self.T_old, self.Tb, self.gT, self.bT, self.diagT, self.diagT_r,
+                      self.offT, self.F, self.Fb, self.scal, *vars(self.fv).values(), *vars(self).values()]:
+                if isinstance(t, torch.Tensor) and t.is_cuda:
+                    t.record_stream(es)
             r = nat.FtmiRed()
             r.partials, r.counter = nat.ptr(buf["red_partials"]), nat.ptr(buf["red_counter"])
             buf["c_red"] = r
             self._ebuf = buf
-            self._estream = torch.cuda.Stream(device=dev)
+            self._estream = es
         return self._estream, self._ebuf
 
     def _energy_step_overlapped(self, L, M, B):
