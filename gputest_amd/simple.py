@@ -326,7 +326,7 @@ class SimpleSolver:
             if self._graph is None:
                 self._record_graph()
             self._graph.replay()
-            sc = self.scal.cpu().numpy()  # the one host read of the iteration
+            sc = self._read_scalars()  # the one host read of the iteration
             out = self._replay_infos(sc[_NSCAL:])
             if out is None:
                 self._restore_saved()
@@ -337,6 +337,16 @@ class SimpleSolver:
         self._note_counts(*out)
         self._amg_age += 1
         return self._finish(*out, sc=sc)
+
+    def _read_scalars(self) -> np.ndarray:
+        """The reduced scalars, copied into pinned host memory on the current stream and read
+        after that stream alone has reached them: work queued on the energy stream is not
+        waited for (a copy into pageable memory can wait for the whole device)."""
+        if getattr(self, "_scal_h", None) is None or self._scal_h.shape != self.scal.shape:
+            self._scal_h = torch.empty(self.scal.shape, dtype=self.scal.dtype, pin_memory=True)
+        self._scal_h.copy_(self.scal, non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        return self._scal_h.numpy().copy()
 
     def _iteration_state(self) -> Dict[str, torch.Tensor]:
         """The device arrays an outer iteration reads before it writes them: the fields, the
@@ -563,7 +573,7 @@ class SimpleSolver:
         c, dim = self.case, self.dim
         cs = self._case_sums() if self._parts is not None else None
         if sc is None:
-            sc = self.scal.cpu().numpy()  # the one host read of the iteration
+            sc = self._read_scalars()  # the one host read of the iteration
         self._accumulate_timers()
         res = {}
         num = [sc[_S_MOM + 2 * k] for k in range(dim)]
@@ -686,7 +696,8 @@ class SimpleSolver:
 
     def _energy_side(self):
         """The energy stream (default priority) and the energy solve's own buffers: inverse
-        diagonal, slice-layout operator, reduction partials, Krylov workspace and report."""
+        diagonal, slice-layout operator, the FP32 system in mixed precision, Krylov workspace and
+        report."""
         if self._estream is None:
             dev, nc, nf = self.device, self.nc, self.fv.n_if
             es = torch.cuda.Stream(device=dev)
@@ -698,7 +709,6 @@ class SimpleSolver:
             with torch.cuda.stream(es):
                 buf = {"inv_r": z(nc), "ws": Workspace(dev),
                        "soff": z(sl) if sl is not None and not self.mixed else None,
-                       "red_partials": z(2048 * 8), "red_counter": torch.zeros(64, dtype=torch.int32, device=dev),
                        "report": z(4)}
                 if self.mixed:
                     buf.update(diag32=f(nc), diag_r32=f(nc), off32=f(2 * nf), inv_r32=f(nc), ex32=f(nc), b32=f(nc),
@@ -710,27 +720,25 @@ class SimpleSolver:
                       self.offT, self.F, self.Fb, self.scal, *vars(self.fv).values(), *vars(self).values()]:
                 if isinstance(t, torch.Tensor) and t.is_cuda:
                     t.record_stream(es)
-            r = nat.FtmiRed()
-            r.partials, r.counter = nat.ptr(buf["red_partials"]), nat.ptr(buf["red_counter"])
-            buf["c_red"] = r
             self._ebuf = buf
             self._estream = es
         return self._estream, self._ebuf
 
     def _energy_step_overlapped(self, L, M, B):
-        """The energy step on the energy stream, after this iteration's corrections. The main
-        stream waits only for the assembly (which reads the corrected fluxes and writes the
-        energy residual that the iteration's host read takes); the solve, every iteration of
-        it enqueued without host reads (the capture mode of the Krylov solvers, one report
-        slot), runs on while the host starts the next iteration. Same kernels, same operands,
-        same results as the serial step; its Krylov info is read when it is first asked for."""
+        """The energy step with its solve on the energy stream. The assembly (boundary values,
+        gradient, system, residual sums) runs on the main stream once the previous solve has
+        finished with the energy buffers (a device-side wait, normally long passed: that solve
+        ran beside this iteration's flow work); the solve, every iteration of it enqueued without
+        host reads (the Krylov solvers' capture mode, one report slot), then runs on the energy
+        stream while the host starts the next iteration. The main stream never waits for the
+        energy stream within an iteration. Same kernels, same operands, same results as the
+        serial step; its Krylov info is read when it is first asked for."""
         main = torch.cuda.current_stream(self.device)
         es, buf = self._energy_side()
-        es.wait_stream(main)  # the fluxes and fields of this iteration's corrections
+        main.wait_stream(es)  # the previous solve is done with T, Tnew and the energy system
+        self._energy_assemble(L, nat.stream_ptr(), M, B, ctypes.byref(self.c_red), buf)
+        es.wait_stream(main)  # this iteration's system
         with torch.cuda.stream(es):
-            s = nat.stream_ptr()
-            self._energy_assemble(L, s, M, B, ctypes.byref(buf["c_red"]), buf)
-            assembled = es.record_event()
             self._energy_solve_unread(L, buf)
             # two host slots: the previous iteration's report is read (in that iteration's
             # successor's host part) while this one may still be on its way
@@ -738,7 +746,6 @@ class SimpleSolver:
             buf["slot"] ^= 1
             buf["report_h"][slot].copy_(buf["report"], non_blocking=True)
             done = es.record_event()
-        main.wait_event(assembled)
         return _PendingSolve(buf["report_h"][slot], done)
 
     def _energy_solve_unread(self, L, buf):
