@@ -714,10 +714,11 @@ class SimpleSolver:
                     buf.update(diag32=f(nc), diag_r32=f(nc), off32=f(2 * nf), inv_r32=f(nc), ex32=f(nc), b32=f(nc),
                                soff32=f(sl) if sl is not None else None)
             buf.update(report_h=torch.zeros(2, 4, dtype=torch.float64, pin_memory=True), slot=0)
-            # the main stream's tensors that the energy stream reads or writes: freed (with the
-            # solver) while an energy solve is still running, their memory waits for it
-            for t in [self._T, self.Tnew, self.T_old, self.Tb, self.gT, self.bT, self.diagT, self.diagT_r,
-                      self.offT, self.F, self.Fb, self.scal, *vars(self.fv).values(), *vars(self).values()]:
+            # the solve reads the energy system, the mesh's slice layout and Tnew, and writes T:
+            # every device tensor of the solver and its mesh is marked as used by the energy
+            # stream, so that memory freed with them (a solver dropped while its last solve
+            # still runs) is not handed out again before that solve ends
+            for t in [*vars(self.fv).values(), *vars(self).values()]:
                 if isinstance(t, torch.Tensor) and t.is_cuda:
                     t.record_stream(es)
             self._ebuf = buf
