@@ -99,10 +99,11 @@ def test_rebuild_of_the_solvers_pressure_system_equals_a_full_set_up():
 
 
 def test_solver_reuses_a_hierarchy_only_for_its_exact_matrix():
-    """The solver's hierarchy bookkeeping (``SimpleSolver._next_hierarchy``) over a fixed
-    sequence of pressure matrices: a hierarchy is used as it is only for the matrix it was set up
-    for, bit for bit; the current one is rebuilt when the finest strength mask stays, and a new
-    one is set up (keeping the current one) when it changes. After every step the hierarchy in
+    """The solver's hierarchy bookkeeping (``SimpleSolver._reuse_hierarchy``, then
+    ``_next_hierarchy``) over a fixed sequence of pressure matrices: a hierarchy is used as it is
+    only for the matrix it was set up for, bit for bit; the current one is rebuilt when the finest
+    strength mask stays, and a new one is set up (keeping the current one) when it changes.
+    After every step the hierarchy in
     use equals a full set-up of the matrix."""
     from gputest_amd import case as C
     from gputest_amd.simple import SimpleSolver
@@ -125,7 +126,7 @@ def test_solver_reuses_a_hierarchy_only_for_its_exact_matrix():
         diag, off = _laplacian(fv, a)
         s.diag_p.copy_(diag)
         s.offp.copy_(off)
-        how = s._next_hierarchy(fv)
+        how = (s._reuse_hierarchy(fv) if s._amg is not None else None) or s._next_hierarchy(fv)
         got.append(how)
         _assert_identical(s._amg, SmoothedAggregationAMG.from_ldu(fv, diag, off, singular=s.singular_p))
         assert s._amg.matches_ldu(fv, s.diag_p, s.offp)
