@@ -12,7 +12,8 @@ after iteration ``--steady-from``. ``mixed`` times the mixed-precision solver (F
 assembly and BiCGSTAB corrections, FP32 AMG-PCG refined in FP64) over the same windows.
 ``equal_error`` runs both solvers to convergence on the 250 x 125 duct and compares them with
 the reference's converged FP64 solution. ``batch`` times 256 cavities in one launch,
-``adjoint`` the drag gradient on a square cylinder.
+``adjoint`` the drag gradient on a square cylinder. ``long_run`` (``--long-run SECONDS``, off by
+default) runs both precisions from rest towards convergence at full size for that long each.
 
 Prints one JSON line (see ``--help``). ``roofline`` is measured live for the dominant kernel,
 the BiCGSTAB p-update fused with the face-operator SpMV (its launches in three further outer
@@ -557,6 +558,55 @@ def equal_error_leg(nx=250, ny=125):
     return out
 
 
+def long_run_leg(mesh, nx, ny, seconds):
+    """How far the workload gets towards convergence at full size. The FP64 solver runs from
+    rest for ``seconds`` (or to scaled residuals of 1e-8); the mixed solver then runs as many
+    iterations. Reported per precision: the rate over the whole run, the largest scaled residual
+    every 30 s, the relaxation the stability guard left, and how far u still is from the fully
+    developed profile 4 y (1 - y) that the parabolic inlet makes the exact solution; and the
+    relative L2 difference of the two end states, after the same iterations."""
+    import numpy as np
+    import torch
+
+    out, ends, until = {"seconds_fp64": seconds}, {}, None
+    for precision in ("fp64", "mixed"):
+        _, case, _ = _workload(nx, ny, precision)
+        case.tol, case.max_iter = 1e-8, (10 ** 9 if until is None else until)
+        s = _solver(mesh, case)
+        y = s.cell_centres()[:, 1]
+        hist = []
+        torch.cuda.synchronize()
+        t0 = last = time.perf_counter()
+        while not s.finished:
+            s.step()
+            now = time.perf_counter()
+            if now - last >= 30.0 or (until is None and now - t0 >= seconds):
+                r = max(s.residuals()["scaled"].values())
+                hist.append([s.iteration, round(now - t0, 1), float(f"{r:.3g}")])
+                print(f"[bench:long-run] {precision} {hist[-1]}", file=sys.stderr, flush=True)
+                last = now
+                if until is None and now - t0 >= seconds:
+                    break
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        U = s.U.cpu().numpy()
+        out[precision] = {"iterations": s.iteration, "seconds": round(dt, 1), "it_per_s": round(s.iteration / dt, 2),
+                          "converged": bool(s.converged), "guard_events": len(s.guard_events),
+                          "settings": {"algorithm": s.algo, "relax_u": round(float(s.relax_u), 4),
+                                       "relax_p": round(float(s.relax_p), 4)},
+                          "residuals": {k: float(f"{v:.3g}") for k, v in s.residuals()["scaled"].items()},
+                          "u_vs_developed_profile": _rel(U[0], 4.0 * y * (1.0 - y)),
+                          "history": hist}
+        ends[precision] = (U, s.p.cpu().numpy(), s.T.cpu().numpy())
+        until = s.iteration
+        del s
+    (U0, p0, T0), (U1, p1, T1) = ends["fp64"], ends["mixed"]
+    out["mixed_vs_fp64_end"] = {"U": _rel(U1, U0), "p": _rel(p1 - np.mean(p1), p0 - np.mean(p0)),
+                                "T-T_in": _rel(T1 - 300.0, T0 - 300.0)}
+    out["speedup_vs_fp64"] = round(out["mixed"]["it_per_s"] / out["fp64"]["it_per_s"], 3)
+    return out
+
+
 def run_timed(mesh, case, warmup, steps, tag):
     """Warm-up, then `steps` timed outer iterations bracketed by device synchronisation."""
     import torch
@@ -641,6 +691,8 @@ def main():
                     help="1: each iteration's energy step runs beside the next iteration's flow work (default); "
                          "0: serial, as for contention-free kernel statistics")
     ap.add_argument("--equal-error", type=int, default=1, help="run the converged equal-error comparison (1) or not")
+    ap.add_argument("--long-run", type=float, default=0.0, help="also run FP64 and mixed from rest for this many "
+                    "seconds each towards convergence and compare their end states (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mixed", type=int, default=1, help="also time the mixed-precision solver (1) or not (0)")
     ap.add_argument("--batch", type=int, default=256, help="also time a batch of this many cavities (256: the "
@@ -703,6 +755,8 @@ def main():
         out["sst"] = sst_leg(4 * args.nx, args.ny // 2, args.steps, args.warmup)
     if args.channel:
         out["channel"] = channel_leg(128, 64, 64, 200)
+    if args.long_run > 0:
+        out["long_run"] = long_run_leg(mesh, args.nx, args.ny, args.long_run)
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"], ref = cpu_baseline(args.nx, args.ny, args.cpu_warmup, args.cpu_iters)
