@@ -119,7 +119,8 @@ def measure_roofline(s, steps=3):
 
 def pcg_iteration_bytes(s) -> int:
     """Compulsory HBM bytes of one AMG-PCG iteration of the pressure solve, from the solver's
-    live hierarchy (FP64 values, int32 indices): pcg_ap (face operator + z, p_prev in, p, q out),
+    live hierarchy (FP64 values, int32 indices): pcg_ap (face operator + z, p_prev in, p, q out;
+    in the 64-row slice layout the row extents are a base per slice and a degree byte per row),
     pcg_xr (p, q, x, r in; x, r out), with the projection pcg_z (r, z; without it r.z comes from
     the last smoothing sweep, which reads both anyway), and one V(2,2) cycle. Per level of n rows
     and nnz(A): two fused pre-smoothing sweeps and the residual read A, D^-1 / b / x and write
@@ -127,7 +128,8 @@ def pcg_iteration_bytes(s) -> int:
     the coarse x and x and writes x; the two post-smoothing sweeps read A, D^-1, b, x and
     write y. The dense coarse solve reads its inverse."""
     n, nf = s.nc, s.fv.n_if
-    b = 8 * n + 4 * (n + 1) + 24 * nf + 32 * n  # pcg_ap
+    rows = (n + 63) // 64 * 4 + n if s.soffp is not None else 4 * (n + 1)
+    b = 8 * n + rows + 24 * nf + 32 * n  # pcg_ap
     b += 48 * n  # pcg_xr
     if s.singular_p:
         b += 16 * n  # pcg_z (mean removal and r.z)
@@ -159,13 +161,15 @@ def pressure_solve_roofline(s, info, iters=100) -> dict:
 
     torch.cuda.synchronize()
     x = torch.zeros_like(s.rhs_p)
-    pcg_ldu(s.fv, s.diag_p, s.offp, s.rhs_p, x, 1e-30, 3, s._amg, s.singular_p)  # first use
+    # the solver's own operator layout (its sliced copy holds the same system)
+    kw = {"sliced_off": s.soffp}
+    pcg_ldu(s.fv, s.diag_p, s.offp, s.rhs_p, x, 1e-30, 3, s._amg, s.singular_p, **kw)  # first use
     ts = []
     for _ in range(3):
         x.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        pinfo = pcg_ldu(s.fv, s.diag_p, s.offp, s.rhs_p, x, 1e-30, iters, s._amg, s.singular_p)
+        pinfo = pcg_ldu(s.fv, s.diag_p, s.offp, s.rhs_p, x, 1e-30, iters, s._amg, s.singular_p, **kw)
         torch.cuda.synchronize()
         ts.append((time.perf_counter() - t0) / max(pinfo.iterations, 1))
     t = sorted(ts)[1]  # the median of three solves

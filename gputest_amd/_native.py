@@ -10,7 +10,7 @@ import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libftmi.so")
-ABI_VERSION = 106
+ABI_VERSION = 107
 
 P = ctypes.c_void_p
 I32, I64, F64 = ctypes.c_int, ctypes.c_int64, ctypes.c_double
@@ -87,6 +87,7 @@ _SIGS = {
     "ftmi_bicgstab_ldu": ([I64, P, P, P, P, P, P, P, F64, I64, I32, P, P, P, P], I32),
     "ftmi_pcg_work_bytes": ([I64], I64),
     "ftmi_pcg_ldu": ([I64, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
+    "ftmi_pcg_ldu_sliced": ([I64, P, P, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
     "ftmi_pcg_csr": ([I64, P, P, P, P, P, F64, I64, P, I32, P, P, P, P], I32),
     "ftmi_amg_create": ([F64, I32], P),
     "ftmi_amg_destroy": ([P], None),

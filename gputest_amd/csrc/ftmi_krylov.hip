@@ -1769,6 +1769,17 @@ int ftmi_pcg_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_
     return pcg_run<LduOp<double>, double>(n, A, b, x, tol, maxiter, M, project, work, iters, res, (hipStream_t)stream);
 }
 
+int ftmi_pcg_ldu_sliced(int64_t n, const double* diag, const int32_t* base, const uint8_t* deg, const int32_t* nbr,
+                        const double* off, const double* b, double* x, double tol, int64_t maxiter, void* amg,
+                        int project, void* work, int64_t* iters, double* res, void* stream) {
+    Amg<double>* M = nullptr;
+    if (amg && !(M = amg_of<double>(amg, "ftmi_pcg_ldu_sliced: the preconditioner is not an FP64 hierarchy")))
+        return -1;
+    LduSellOp<double> A{diag, base, deg, nbr, off};
+    return pcg_run<LduSellOp<double>, double>(n, A, b, x, tol, maxiter, M, project, work, iters, res,
+                                              (hipStream_t)stream);
+}
+
 int ftmi_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
                  double* x, double tol, int64_t maxiter, void* amg, int project, void* work, int64_t* iters,
                  double* res, void* stream) {

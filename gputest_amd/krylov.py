@@ -90,10 +90,16 @@ def bicgstab_ldu(fv, diag, off, b, inv_diag, x, tol: float, maxiter: int, relati
                 nat.ptr(work))
 
 
-def pcg_ldu(fv, diag, off, b, x, tol: float, maxiter: int, amg=None, project=False, ws=None):
-    """PCG from zero into ``x``, preconditioned by one V-cycle of ``amg`` (``krylov.py:25-65``)."""
+def pcg_ldu(fv, diag, off, b, x, tol: float, maxiter: int, amg=None, project=False, ws=None, sliced_off=None):
+    """PCG from zero into ``x``, preconditioned by one V-cycle of ``amg`` (``krylov.py:25-65``).
+    With ``sliced_off`` (``off`` already in ``fv``'s slice layout, ``fv.slice_values``) the
+    operator products read the sliced half-edges: the same sums, coalesced loads."""
     n = b.shape[0]
     work = (ws or _ws(b.device)).get("pcg", n)
+    if sliced_off is not None:
+        return _run(nat.lib().ftmi_pcg_ldu_sliced, n, nat.ptr(diag), nat.ptr(fv.sl_base), nat.ptr(fv.sl_deg),
+                    nat.ptr(fv.sl_nbr), nat.ptr(sliced_off), nat.ptr(b), nat.ptr(x), float(tol), int(maxiter),
+                    amg.handle if amg is not None else None, int(bool(project)), nat.ptr(work))
     return _run(nat.lib().ftmi_pcg_ldu, n, nat.ptr(diag), nat.ptr(fv.cptr), nat.ptr(fv.cnbr), nat.ptr(off),
                 nat.ptr(b), nat.ptr(x), float(tol), int(maxiter), amg.handle if amg is not None else None,
                 int(bool(project)), nat.ptr(work))

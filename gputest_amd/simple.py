@@ -235,6 +235,8 @@ class SimpleSolver:
         self.U_old, self.p_old = z(dim, nc), z(nc)
         # the transport operator in the slice layout (FP64 BiCGSTAB), shared by momentum and energy
         self.soff = z(self.fv.sl_size) if self.fv.sl_base is not None and not self.mixed else None
+        # the pressure operator in the same layout (FP64 AMG-PCG)
+        self.soffp = z(self.fv.sl_size) if self.soff is not None else None
         if self.energy:
             self.Tb, self.gT, self.Tnew, self.bT = z(nb), z(nc, dim), z(nc), z(nc)
             self.diagT, self.diagT_r, self.offT, self.T_old = z(nc), z(nc), z(2 * nf), z(nc)
@@ -546,8 +548,10 @@ class SimpleSolver:
             if pinfo.iterations >= 0 and not pinfo.converged:  # (capture mode reports later)
                 self._amg_needs_setup = True
         else:
+            if self.soffp is not None:
+                fv.slice_values(self.offp, self.soffp)
             pinfo = pcg_ldu(fv, self.diag_p, self.offp, self.rhs_p, self.pp, c.pressure_tol, 200, self._amg,
-                            self.singular_p, ws=self.ws)
+                            self.singular_p, ws=self.ws, sliced_off=self.soffp)
         self._mark("pressure")
         # 7. corrections
         nat.check(L.ftmi_correct(M, B, nat.ptr(self.pp), nat.ptr(self.Ustar), nat.ptr(self.dP), nat.ptr(self.Fst),

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define FTMI_ABI_VERSION 106
+#define FTMI_ABI_VERSION 107
 
 /* replaces ft_abi_version (flowtrace/native/fvm_kernels.h:27) */
 int ftmi_abi_version(void);
@@ -142,6 +142,11 @@ int64_t ftmi_pcg_work_bytes(int64_t n);
 int ftmi_pcg_ldu(int64_t n, const double* diag, const int32_t* ptr, const int32_t* nbr, const double* off,
                  const double* b, double* x, double tol, int64_t maxiter, void* amg, int project, void* work,
                  int64_t* iters, double* res, void* stream);
+/* new: ftmi_pcg_ldu with the operator's half-edges in the slice layout of
+ * ftmi_bicgstab_ldu_sliced (the same algorithm and sums, coalesced loads) */
+int ftmi_pcg_ldu_sliced(int64_t n, const double* diag, const int32_t* base, const uint8_t* deg, const int32_t* nbr,
+                        const double* off, const double* b, double* x, double tol, int64_t maxiter, void* amg,
+                        int project, void* work, int64_t* iters, double* res, void* stream);
 int ftmi_pcg_csr(int64_t n, const int32_t* indptr, const int32_t* indices, const double* data, const double* b,
                  double* x, double tol, int64_t maxiter, void* amg, int project, void* work, int64_t* iters,
                  double* res, void* stream);

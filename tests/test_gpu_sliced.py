@@ -1,14 +1,17 @@
-"""The face operator in the 64-row slice layout (``ftmi_bicgstab_ldu_sliced``): the same sums in
-the same order as the row layout, so the BiCGSTAB solve is bit-identical, on meshes whose
-slices mix widths (boundary rows, quads and triangles, a cell count that is not a multiple
-of 64)."""
+"""The face operator in the 64-row slice layout (``ftmi_bicgstab_ldu_sliced``,
+``ftmi_pcg_ldu_sliced``): the same sums in the same order as the row layout, so the BiCGSTAB and
+AMG-PCG solves are bit-identical, on meshes whose slices mix widths (boundary rows, quads and
+triangles, a cell count that is not a multiple of 64)."""
 import numpy as np
 import pytest
 import torch
 
+from gputest_amd import case as C
 from gputest_amd.fvmesh import FVMesh
-from gputest_amd.krylov import bicgstab_ldu
-from gputest_amd.mesh import cartesian_mesh_2d, extrude_to_3d
+from gputest_amd.krylov import bicgstab_ldu, pcg_ldu
+from gputest_amd.mesh import cartesian_mesh_2d, duct_mesh, extrude_to_3d
+from gputest_amd.simple import SimpleSolver
+from tests.golden_cases import duct_case
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
@@ -29,6 +32,31 @@ def test_sliced_bicgstab_is_bit_identical(mesh):
     xa, xb = x0.clone(), x0.clone()
     ia = bicgstab_ldu(fv, diag, off, b, inv, xa, 1e-10, 200, True)
     ib = bicgstab_ldu(fv, diag, off, b, inv, xb, 1e-10, 200, True, sliced_off=fv.slice_values(off))
+    assert ia.iterations == ib.iterations and ia.converged and ib.converged
+    assert ia.residual == ib.residual
+    assert torch.equal(xa, xb)
+
+
+@pytest.mark.parametrize("which", ["cavity", "duct"])
+def test_sliced_pcg_is_bit_identical(which):
+    """The solver's pressure systems, singular (closed cavity) and not (duct), solved by AMG-PCG
+    on the row layout and on the slice layout: the same iterations, residual and solution."""
+    if which == "cavity":
+        mesh = cartesian_mesh_2d(37, 29, triangulate=True, warp=0.05)
+        case = C.UnstructuredCase(boundaries={"top": C.BoundarySpec("wall", velocity=[1.0, 0.0])}, viscosity=0.01,
+                                  max_iter=3, tol=0.0)
+    else:
+        mesh, case = duct_mesh(60, 30), duct_case(C, max_iter=3)
+    s = SimpleSolver(mesh, case, DEV)
+    while not s.finished:
+        s.step()
+    assert s.soffp is not None and s._amg is not None and len(s._amg.levels) >= 1
+    assert s.singular_p == (which == "cavity")
+    out = []
+    for kw in ({}, {"sliced_off": s.fv.slice_values(s.offp)}):
+        x = torch.zeros_like(s.rhs_p)
+        out.append((pcg_ldu(s.fv, s.diag_p, s.offp, s.rhs_p, x, 1e-10, 200, s._amg, s.singular_p, **kw), x))
+    (ia, xa), (ib, xb) = out
     assert ia.iterations == ib.iterations and ia.converged and ib.converged
     assert ia.residual == ib.residual
     assert torch.equal(xa, xb)
