@@ -182,21 +182,24 @@ __global__ void __launch_bounds__(kBlock) k_tentative(int64_t n, const int32_t* 
 // workgroup), then y = (D^-1 A) x with the products (d_i a_ij) x_j of a row-by-row sparse
 // product, and the partials of ||y||^2 and ||x||^2. The first iteration takes x as given.
 // Rows are handled by TPR lanes (rows_tpr): 1 keeps the column-order sum, wider groups sum in a
-// fixed lane tree.
-template <int TPR>
-__global__ void __launch_bounds__(kBlock) k_pow_step(int64_t n, const int32_t* __restrict__ ptr,
-                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
-                                                     const double* __restrict__ dinv, const double* __restrict__ y_prev,
-                                                     double* __restrict__ x, double* __restrict__ y, int first,
-                                                     const double* part_in, int np_in, double* part_out) {
+// fixed lane tree. The first iteration is a kernel of its own (k_pow_first): a run-time flag
+// put a branch, and a wait for its load, inside every gather of a row.
+template <int TPR, bool FIRST>
+__device__ inline void pow_step(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                const double* __restrict__ val, const double* __restrict__ dinv,
+                                const double* __restrict__ y_prev, double* __restrict__ x, double* __restrict__ y,
+                                const double* part_in, int np_in, double* part_out) {
     double scale = 1.0;
-    if (!first) {
+    if (!FIRST) {
         double tot[2];
         consume_sum<2>(part_in, np_in, tot);
         const double ny = sqrt(tot[0]);
         scale = ny > 1e-300 ? ny : 1e-300;
     }
-    auto xv = [&](int64_t j) { return first ? x[j] : y_prev[j] / scale; };
+    auto xv = [&](int64_t j) {
+        if (FIRST) return x[j];
+        return y_prev[j] / scale;
+    };
     const int g = gridDim.x, b = blockIdx.x;
     int logical = b;
     if (g >= kNXcd && g % kNXcd == 0) logical = (b % kNXcd) * (g / kNXcd) + b / kNXcd;
@@ -215,13 +218,31 @@ __global__ void __launch_bounds__(kBlock) k_pow_step(int64_t n, const int32_t* _
         for (int off = TPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, TPR);
         if (lane == 0) {
             const double xi = xv(i);
-            if (!first) x[i] = xi;
+            if (!FIRST) x[i] = xi;
             y[i] = s;
             acc[0] += s * s;
             acc[1] += xi * xi;
         }
     }
     block_partials<2>(acc, part_out);
+}
+
+template <int TPR>
+__global__ void __launch_bounds__(kBlock) k_pow_first(int64_t n, const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                      const double* __restrict__ dinv, const double* __restrict__ y_prev,
+                                                      double* __restrict__ x, double* __restrict__ y,
+                                                      const double* part_in, int np_in, double* part_out) {
+    pow_step<TPR, true>(n, ptr, col, val, dinv, y_prev, x, y, part_in, np_in, part_out);
+}
+
+template <int TPR>
+__global__ void __launch_bounds__(kBlock) k_pow_step(int64_t n, const int32_t* __restrict__ ptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ dinv, const double* __restrict__ y_prev,
+                                                     double* __restrict__ x, double* __restrict__ y,
+                                                     const double* part_in, int np_in, double* part_out) {
+    pow_step<TPR, false>(n, ptr, col, val, dinv, y_prev, x, y, part_in, np_in, part_out);
 }
 
 // lam = ||y|| / max(||x||, 1e-300)
@@ -824,8 +845,10 @@ int ftmi_amgs_power(int64_t n, int64_t nnz, const int32_t* ptr, const int32_t* c
     double* yb[2] = {y, y + n};
     for (int it = 0; it < iters; ++it) {
         const int a = it & 1, c = a ^ 1;
-        ROWS_TPR_LAUNCH(tpr, k_pow_step, g, s, n, ptr, col, val, dinv, yb[c], x, yb[a], it == 0, part[c], g,
-                        part[a]);
+        if (it == 0)
+            ROWS_TPR_LAUNCH(tpr, k_pow_first, g, s, n, ptr, col, val, dinv, yb[c], x, yb[a], part[c], g, part[a]);
+        else
+            ROWS_TPR_LAUNCH(tpr, k_pow_step, g, s, n, ptr, col, val, dinv, yb[c], x, yb[a], part[c], g, part[a]);
     }
     k_pow_fin<<<1, kBlock, 0, s>>>(part[(iters - 1) & 1], g, &out[0]);
     FTMI_LAUNCH_CHECK("ftmi_amgs_power");

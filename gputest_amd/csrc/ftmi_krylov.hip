@@ -1388,19 +1388,21 @@ __global__ void __launch_bounds__(kBlock) pcg_z(int64_t n, const T* __restrict__
     block_partials<1>(acc, part_out);
 }
 
-// rz_new from the partials; first: p = z, rz = rz_new; else breakdown if rz == 0, beta =
-// rz_new / rz, p = z + beta p_prev; then q = A p and partials p.q
-template <class Op, typename T>
+// rz_new from the partials; FIRST: p = z, rz = rz_new; else breakdown if rz == 0, beta =
+// rz_new / rz, p = z + beta p_prev; then q = A p and partials p.q. FIRST is a template
+// argument: as a run-time flag it put a branch, and a wait for its load, inside every
+// neighbour gather, so the gathers of a row no longer overlapped.
+template <class Op, typename T, bool FIRST>
 __global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const T* __restrict__ z, const T* __restrict__ p_prev,
                                                  T* __restrict__ p, T* __restrict__ q, State* S, const double* part_in,
-                                                 int np_in, double* part_out, int first, int k) {
+                                                 int np_in, double* part_out, int k) {
     if (S->st[I_DONE]) return;
     double tot[1];
     consume_sum<1>(part_in, np_in, tot);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     const int cur = S_RZ + (k & 1), prev = S_RZ + ((k - 1) & 1);
     double beta_d = 0.0;
-    if (!first) {
+    if (!FIRST) {
         const double rz = S->sc[prev];
         if (rz == 0.0) {  // breakdown found at the end of the previous iteration
             if (lead) {
@@ -1414,7 +1416,10 @@ __global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const T* __res
     }
     if (lead) S->sc[cur] = tot[0];
     const T beta = (T)beta_d;
-    auto pf = [&](int64_t j) { return first ? z[j] : z[j] + beta * p_prev[j]; };
+    auto pf = [&](int64_t j) {
+        if (FIRST) return z[j];
+        return z[j] + beta * p_prev[j];
+    };
     int64_t lo, hi;
     block_range(n, lo, hi);
     T acc[1] = {T(0)};
@@ -1580,11 +1585,17 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
     };
     int nrz = precond(kNoFinish);
     FTMI_LAUNCH_CHECK("pcg init");
+    auto launch_ap = [&](int64_t k, int a, int c) {
+        if (k == 1)
+            pcg_ap<Op, T, true><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, nrz, P1, (int)k);
+        else
+            pcg_ap<Op, T, false><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, nrz, P1, (int)k);
+    };
     if (g_capture) {
         if (cap < 0) report = capture_slot(maxiter, &cap);
         for (int64_t k = 1; k <= cap; ++k) {
             const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-            pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, nrz, P1, k == 1, (int)k);
+            launch_ap(k, a, c);
             pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
             nrz = precond(PcgFinish{W.S, P2, g, tol, tol_dev, (int)k, kNoPost});
         }
@@ -1597,7 +1608,7 @@ int pcg_run(int64_t n, Op A, const T* b, XT* x, double tol, int64_t maxiter, Amg
     int last = 0;
     for (int64_t k = 1; k <= maxiter; ++k) {
         const int a = (int)((k - 1) & 1), c = (int)(k & 1);
-        pcg_ap<Op, T><<<g, kBlock, 0, st>>>(n, A, W.z, W.p[a], W.p[c], W.q, W.S, P0, nrz, P1, k == 1, (int)k);
+        launch_ap(k, a, c);
         pcg_xr<T, XT><<<g, kBlock, 0, st>>>(n, W.p[c], W.q, x, W.r, W.S, P1, g, P2, (int)k);
         nrz = precond(PcgFinish{W.S, P2, g, tol, tol_dev, (int)k, next_post(pn, last ^ 1)});
         FTMI_LAUNCH_CHECK("pcg iteration");
