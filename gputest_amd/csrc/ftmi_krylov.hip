@@ -371,10 +371,13 @@ template <class Op, typename T>
 __global__ void __launch_bounds__(kBlock) bi_st(int64_t n, Op A, const T* __restrict__ r, const T* __restrict__ v,
                                                 const T* __restrict__ inv, T* __restrict__ s, T* __restrict__ t,
                                                 State* S, const double* part_in, int np_in, double* part_out, int k) {
-    if (S->st[I_DONE]) return;
+    // the state words this kernel reads are loaded together with the done flag, ahead of the
+    // partial sums, so their latencies overlap
+    const int done = S->st[I_DONE];
+    const double rho = S->sc[S_RHONEW];
+    if (done) return;
     double tot[1];
     consume_sum<1>(part_in, np_in, tot);
-    const double rho = S->sc[S_RHONEW];
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (tot[0] == 0.0) {  // breakdown: the iteration stops before its first update
         if (lead) {
@@ -416,11 +419,14 @@ __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const T* __restrict__ 
                                                const T* __restrict__ rh, T* __restrict__ r, T* __restrict__ x0,
                                                T* __restrict__ x1, State* S, const double* part_in, int np_in,
                                                double* part_out, double tol, int k) {
-    if (S->st[I_DONE]) return;
+    const int done = S->st[I_DONE];
+    const double bnorm = S->sc[S_BNORM], alpha_d = S->sc[S_ALPHA];
+    const int cur = S->st[I_CUR];
+    if (done) return;
     double tot[3];
     consume_sum<3>(part_in, np_in, tot);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const double snorm = sqrt(tot[0]), bnorm = S->sc[S_BNORM];
+    const double snorm = sqrt(tot[0]);
     if (snorm / bnorm < tol) {  // x = x_cur + alpha phat, applied after the loop
         if (lead) {
             S->st[I_DONE] = 1;
@@ -434,8 +440,7 @@ __global__ void __launch_bounds__(kBlock) bi_r(int64_t n, const T* __restrict__ 
     const double om_d = tot[1] > 0.0 ? tot[2] / tot[1] : 0.0;
     if (lead) S->sc[S_OMEGA] = om_d;
     const T om = (T)om_d;
-    const T alpha = (T)S->sc[S_ALPHA];
-    const int cur = S->st[I_CUR];
+    const T alpha = (T)alpha_d;
     const T* xc = cur ? x1 : x0;
     T* xo = cur ? x0 : x1;
     int64_t lo, hi;
@@ -1396,14 +1401,15 @@ template <class Op, typename T, bool FIRST>
 __global__ void __launch_bounds__(kBlock) pcg_ap(int64_t n, Op A, const T* __restrict__ z, const T* __restrict__ p_prev,
                                                  T* __restrict__ p, T* __restrict__ q, State* S, const double* part_in,
                                                  int np_in, double* part_out, int k) {
-    if (S->st[I_DONE]) return;
+    const int cur = S_RZ + (k & 1), prev = S_RZ + ((k - 1) & 1);
+    const int done = S->st[I_DONE];
+    const double rz = FIRST ? 0.0 : S->sc[prev];
+    if (done) return;
     double tot[1];
     consume_sum<1>(part_in, np_in, tot);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    const int cur = S_RZ + (k & 1), prev = S_RZ + ((k - 1) & 1);
     double beta_d = 0.0;
     if (!FIRST) {
-        const double rz = S->sc[prev];
         if (rz == 0.0) {  // breakdown found at the end of the previous iteration
             if (lead) {
                 S->st[I_DONE] = 1;
@@ -1439,7 +1445,9 @@ template <typename T, typename XT>
 __global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const T* __restrict__ p, const T* __restrict__ q,
                                                  XT* __restrict__ x, T* __restrict__ r, State* S,
                                                  const double* part_in, int np_in, double* part_out, int k) {
-    if (S->st[I_DONE]) return;
+    const int done = S->st[I_DONE];
+    const double rz = S->sc[S_RZ + (k & 1)];
+    if (done) return;
     double tot[1];
     consume_sum<1>(part_in, np_in, tot);
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -1451,7 +1459,7 @@ __global__ void __launch_bounds__(kBlock) pcg_xr(int64_t n, const T* __restrict_
         }
         return;
     }
-    const double a_d = S->sc[S_RZ + (k & 1)] / tot[0];
+    const double a_d = rz / tot[0];
     const T a = (T)a_d;
     const XT ax = (XT)a_d;
     int64_t lo, hi;
