@@ -276,8 +276,14 @@ __global__ void __launch_bounds__(kBlock) fin_bi_init(const double* partials, in
 // becomes current when ||r|| is finite (else the solve stops on the last finite one), then
 // the convergence test, omega = 0 and rho = 0 breakdowns. Every thread that calls it gets the
 // same answer (true: the solve is done); only `lead` writes the state block.
+// State words it reads that no kernel of the iteration in flight writes (loaded by the caller,
+// ahead of the partial sums)
+struct BiScal {
+    double bnorm, babs, omega;
+};
+
 __device__ inline bool bi_finish_iteration(State* S, const double (&tot)[2], double tol, int k, int maxiter,
-                                           bool lead) {
+                                           bool lead, const BiScal& v) {
     double* sc = S->sc;
     int* st = S->st;
     const double rnorm = sqrt(tot[0]);
@@ -290,16 +296,16 @@ __device__ inline bool bi_finish_iteration(State* S, const double (&tot)[2], dou
         }
         return true;
     }
-    const double bnorm = sc[S_BNORM];
+    const double bnorm = v.bnorm;
     if (lead) {
         st[I_CUR] ^= 1;
         sc[S_RNORM] = rnorm;
         sc[S_RES] = rnorm / bnorm;
     }
     int done = 0, conv = 0, iters = k;
-    if (rnorm / bnorm < tol || rnorm <= 1e-14 * sc[S_BABS]) {
+    if (rnorm / bnorm < tol || rnorm <= 1e-14 * v.babs) {
         done = conv = 1;
-    } else if (sc[S_OMEGA] == 0.0) {
+    } else if (v.omega == 0.0) {
         done = 1;
     } else {
         if (lead) sc[S_RHONEW] = tot[1];
@@ -321,7 +327,8 @@ __device__ inline bool bi_finish_iteration(State* S, const double (&tot)[2], dou
 __global__ void __launch_bounds__(kBlock) fin_bi_r(const double* partials, int nparts, State* S, double tol, int k,
                                                    int maxiter, Post post) {
     double tot[2];
-    if (!S->st[I_DONE] && finalize_sum<2>(partials, nparts, tot)) bi_finish_iteration(S, tot, tol, k, maxiter, true);
+    if (!S->st[I_DONE] && finalize_sum<2>(partials, nparts, tot))
+        bi_finish_iteration(S, tot, tol, k, maxiter, true, BiScal{S->sc[S_BNORM], S->sc[S_BABS], S->sc[S_OMEGA]});
     post_host(S, post);
 }
 
@@ -336,22 +343,27 @@ __global__ void __launch_bounds__(kBlock) bi_pv(int64_t n, Op A, const T* __rest
                                                 const double* part_in, int np_in, double tol, int k, int maxiter,
                                                 Post post, double* part_out) {
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (S->st[I_DONE]) {
+    // every state word read here is loaded with the done flag, ahead of the partial sums, so
+    // their latencies overlap (the finish of the previous iteration writes none of them)
+    const double* sc = S->sc;
+    const int done0 = S->st[I_DONE];
+    double rhonew = sc[S_RHONEW];
+    const double rho = sc[S_RHO], alpha = sc[S_ALPHA];
+    const BiScal fs{sc[S_BNORM], sc[S_BABS], sc[S_OMEGA]};
+    if (done0) {
         if (FIN && lead) post_host(S, post);
         return;
     }
-    const double* sc = S->sc;
-    double rhonew = sc[S_RHONEW];
     if (FIN) {
         double tot[2];
         consume_sum<2>(part_in, np_in, tot);
-        const bool done = bi_finish_iteration(S, tot, tol, k - 1, maxiter, lead);
+        const bool done = bi_finish_iteration(S, tot, tol, k - 1, maxiter, lead, fs);
         if (lead) post_host(S, post);
         if (done) return;
         rhonew = tot[1];
     }
-    const T beta = (T)((rhonew / sc[S_RHO]) * (sc[S_ALPHA] / sc[S_OMEGA]));
-    const T om = (T)sc[S_OMEGA];
+    const T beta = (T)((rhonew / rho) * (alpha / fs.omega));
+    const T om = (T)fs.omega;
     auto phat = [&](int64_t j) { return inv[j] * (r[j] + beta * (p_prev[j] - om * v_prev[j])); };
     int64_t lo, hi;
     block_range(n, lo, hi);
