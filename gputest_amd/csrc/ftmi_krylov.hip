@@ -856,15 +856,18 @@ struct PcgFinish {
 __device__ inline bool pcg_finish(const PcgFinish& f) {
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     State* S = f.S;
-    if (S->st[I_DONE]) {
+    // the state words and the tolerance are loaded with the done flag, ahead of the partial sums
+    const int done0 = S->st[I_DONE];
+    const double bnorm = S->sc[S_BNORM];
+    const double tol = f.tol_dev ? *f.tol_dev : f.tol;
+    if (done0) {
         if (lead) post_host(S, f.post);
         return true;
     }
     double tot[1];
     consume_sum<1>(f.part, f.np, tot);
-    const double tol = f.tol_dev ? *f.tol_dev : f.tol;
     const double rn = sqrt(tot[0]);
-    const double res = rn / S->sc[S_BNORM];
+    const double res = rn / bnorm;
     const bool done = res < tol;
     if (lead) {
         S->sc[S_RNORM] = rn;
